@@ -1,0 +1,241 @@
+// C ABI of the engine (include/gsdr.h): records, device context, HBM store,
+// plain per-round clustering and the pair-level alignment entry.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gsdr.h"
+#include "driver.hpp"
+#include "host.hpp"
+#include "kernels_impl.hpp"
+
+using namespace gsdr;
+
+// ------------------------------------------------------------ error state
+static thread_local std::string g_last_error;
+
+namespace gsdr {
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return GSDR_E_DEVICE;
+}
+int host_threads() {
+  unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc, 16u));
+}
+}  // namespace gsdr
+
+#define HIP_TRY(expr, what)                         \
+  do {                                              \
+    hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+extern "C" {
+
+const char* gsdr_strerror(int32_t status) {
+  switch (status) {
+    case GSDR_OK: return "ok";
+    case GSDR_E_INVALID: return "invalid argument";
+    case GSDR_E_IO: return "i/o error";
+    case GSDR_E_INPUT: return "malformed or unsupported input";
+    case GSDR_E_DEVICE: return "HIP device error";
+    case GSDR_E_NOMEM: return "out of memory";
+    case GSDR_E_LIMIT: return "size limit exceeded";
+    case GSDR_E_BUSY: return "no free slot or memory";
+    case GSDR_E_NOTFOUND: return "unknown job";
+    default: return "unknown status";
+  }
+}
+const char* gsdr_last_error(void) { return g_last_error.c_str(); }
+
+// ----------------------------------------------------------------- records
+int gsdr_records_read_fasta(const char* path, gsdr_records** out) {
+  if (!path || !out) return fail(GSDR_E_INVALID, "null argument");
+  auto r = std::make_unique<gsdr_records>();
+  std::string err;
+  int st = read_fasta(path, r->r, err);
+  if (st != GSDR_OK) return fail(st, err);
+  *out = r.release();
+  return GSDR_OK;
+}
+
+int gsdr_records_from_buffers(const char* names, const int64_t* name_off, const char* data,
+                              const int64_t* data_off, int64_t count, gsdr_records** out) {
+  if (!out || count < 0 || !data_off || (count > 0 && !data)) return fail(GSDR_E_INVALID, "bad buffers");
+  auto r = std::make_unique<gsdr_records>();
+  for (int64_t i = 0; i < count; ++i) {
+    const char* nm = "";
+    int64_t nl = 0;
+    if (names && name_off) { nm = names + name_off[i]; nl = name_off[i + 1] - name_off[i]; }
+    const int64_t dl = data_off[i + 1] - data_off[i];
+    if (dl < 0 || nl < 0) return fail(GSDR_E_INVALID, "negative record length");
+    r->r.push(nm, nl, data + data_off[i], dl);
+  }
+  *out = r.release();
+  return GSDR_OK;
+}
+
+int gsdr_records_generate(int32_t config, uint64_t seed, int64_t n, double median, gsdr_records** out) {
+  if (!out || config < 1 || config > 4) return fail(GSDR_E_INVALID, "config must be 1..4");
+  auto r = std::make_unique<gsdr_records>();
+  GenSpec spec;
+  spec.config = config;
+  spec.seed = seed;
+  spec.n_override = n;
+  spec.median = median;
+  generate(spec, r->r);
+  *out = r.release();
+  return GSDR_OK;
+}
+
+int gsdr_records_write_fasta(const gsdr_records* r, const char* path, int32_t line_width) {
+  if (!r || !path || line_width < 1) return fail(GSDR_E_INVALID, "bad argument");
+  int st = write_fasta(r->r, path, line_width);
+  if (st != GSDR_OK) return fail(st, std::string("cannot write ") + path);
+  return GSDR_OK;
+}
+
+int gsdr_records_sort(gsdr_records* r, int64_t* perm_out) {
+  if (!r) return fail(GSDR_E_INVALID, "null records");
+  std::vector<int64_t> perm = length_order(r->r);
+  r->r = permute(r->r, perm);
+  if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
+  return GSDR_OK;
+}
+
+int64_t gsdr_records_count(const gsdr_records* r) { return r ? r->r.count() : -1; }
+int64_t gsdr_records_bases(const gsdr_records* r) { return r ? (int64_t)r->r.data.size() : -1; }
+
+int gsdr_records_get(const gsdr_records* r, int64_t i, const char** name, int64_t* name_len,
+                     const char** data, int64_t* data_len) {
+  if (!r || i < 0 || i >= r->r.count()) return fail(GSDR_E_INVALID, "record index out of range");
+  if (name) *name = r->r.names.data() + r->r.name_off[i];
+  if (name_len) *name_len = r->r.name_off[i + 1] - r->r.name_off[i];
+  if (data) *data = r->r.data.data() + r->r.data_off[i];
+  if (data_len) *data_len = r->r.data_off[i + 1] - r->r.data_off[i];
+  return GSDR_OK;
+}
+
+int gsdr_records_write_result(const gsdr_records* sorted, const int32_t* cluster, const char* path,
+                              int64_t* reps_out) {
+  if (!sorted || !cluster || !path) return fail(GSDR_E_INVALID, "null argument");
+  int64_t reps = write_result(sorted->r, cluster, path);
+  if (reps < 0) return fail(GSDR_E_IO, std::string("cannot write ") + path);
+  if (reps_out) *reps_out = reps;
+  return GSDR_OK;
+}
+
+int gsdr_records_write_membership(const gsdr_records* sorted, const int64_t* perm,
+                                  const int32_t* cluster, const char* path) {
+  if (!sorted || !cluster || !path) return fail(GSDR_E_INVALID, "null argument");
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(GSDR_E_IO, std::string("cannot write ") + path);
+  const Records& r = sorted->r;
+  for (int64_t s = 0; s < r.count(); ++s) {
+    const std::string nm(r.names.data() + r.name_off[s], (size_t)(r.name_off[s + 1] - r.name_off[s]));
+    fprintf(f, "%lld\t%lld\t%d\t%s\n", (long long)s, (long long)(perm ? perm[s] : s), cluster[s], nm.c_str());
+  }
+  if (fclose(f) != 0) return fail(GSDR_E_IO, "write failed");
+  return GSDR_OK;
+}
+
+void gsdr_records_free(gsdr_records* r) { delete r; }
+
+// ----------------------------------------------------------------- context
+int gsdr_open(int32_t device, gsdr_ctx** out) {
+  if (!out) return fail(GSDR_E_INVALID, "null ctx");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(GSDR_E_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(GSDR_E_INVALID, "device index out of range");
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  auto c = std::make_unique<gsdr_ctx>();
+  c->device = device;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  c->cu_count = prop.multiProcessorCount;
+  c->hbm_bytes = (int64_t)prop.totalGlobalMem;
+  snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    return fail(GSDR_E_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(RoundState) * 4, hipHostMallocDefault), "hipHostMalloc");
+  *out = c.release();
+  return GSDR_OK;
+}
+
+void gsdr_close(gsdr_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->h_state) hipHostFree(c->h_state);
+  delete c;
+}
+
+int gsdr_device_info(gsdr_ctx* c, char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes) {
+  if (!c) return fail(GSDR_E_INVALID, "null ctx");
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
+  if (cu_count) *cu_count = c->cu_count;
+  if (hbm_bytes) *hbm_bytes = c->hbm_bytes;
+  return GSDR_OK;
+}
+
+// ------------------------------------------------------------------- store
+int gsdr_load(gsdr_ctx* c, const gsdr_records* sorted, int32_t word_length, gsdr_store** out) {
+  if (!c || !sorted || !out) return fail(GSDR_E_INVALID, "null argument");
+  if (word_length < 4 || word_length > 8) return fail(GSDR_E_INVALID, "word length must be 4..8");
+  const int64_t n = sorted->r.count();
+  if (n <= 0) return fail(GSDR_E_INPUT, "no records");
+  if (n >= (int64_t)INT32_MAX / 2) return fail(GSDR_E_LIMIT, "too many records");
+  for (int64_t i = 0; i + 1 < n; ++i)
+    if (sorted->r.len(i) < sorted->r.len(i + 1)) return fail(GSDR_E_INVALID, "records are not in length order");
+  auto s = std::make_unique<gsdr_store>();
+  s->ctx = c;
+  int st = store_build(c, sorted->r, word_length, *s);
+  if (st != GSDR_OK) { store_release(*s); return st; }
+  *out = s.release();
+  return GSDR_OK;
+}
+
+void gsdr_store_free(gsdr_store* s) {
+  if (!s) return;
+  store_release(*s);
+  delete s;
+}
+
+int64_t gsdr_store_footprint(const gsdr_store* s) { return s ? s->footprint : -1; }
+
+// -------------------------------------------------------------- clustering
+int gsdr_cluster(gsdr_ctx* c, gsdr_store* s, const gsdr_params* p, int32_t* cluster_out, gsdr_stats* stats) {
+  if (!c || !s || !p || !cluster_out) return fail(GSDR_E_INVALID, "null argument");
+  return cluster_store(c, *s, *p, cluster_out, stats);
+}
+
+int gsdr_cluster_jobs(gsdr_ctx* c, int32_t njobs, gsdr_store* const* stores, const gsdr_params* params,
+                      int32_t* const* cluster_out, gsdr_stats* stats) {
+  if (!c || njobs < 0 || (njobs > 0 && (!stores || !params || !cluster_out)))
+    return fail(GSDR_E_INVALID, "null argument");
+  return cluster_jobs(c, njobs, stores, params, cluster_out, stats);
+}
+
+int gsdr_align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int32_t nseq,
+                     const int32_t* pairs, const int32_t* bc, int32_t npairs, int32_t shape,
+                     int32_t* accept_out, double* cells_exec_out) {
+  if (!c || !data_off || nseq <= 0 || npairs < 0 || (npairs > 0 && (!pairs || !bc || !accept_out)))
+    return fail(GSDR_E_INVALID, "bad argument");
+  return align_pairs(c, data, data_off, nseq, pairs, bc, npairs, shape, accept_out, cells_exec_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// `cluster` command line: drop-in for the reference's main.cpp / checkOption
+// (main.cpp:2-27, func.cpp:2-57). Options are letter/value pairs
+// (`t 0.9 i in.fasta o out.fasta w 6`); `-t/-i/-o/-w` and long options are
+// accepted as well:
+//   --mode plain|batched   launch form (default batched)
+//   --spec K               batched: representatives per speculative step
+//   --membership FILE      also write sorted index, original index, rep, name
+//   --report FILE          JSON report (throughput, filter rates, cells)
+//   --device N             HIP device index
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/gsdr.h"
+#include "host.hpp"
+
+namespace {
+
+[[noreturn]] void usage() {
+  std::cout << "use like this: cluster i inputFile t threshold" << std::endl;
+  std::exit(0);  // the reference exits with status 0 on usage errors (func.cpp:2-5)
+}
+
+struct Options {
+  std::string in = "testData.fasta", out = "result.fasta", membership, report;
+  float t = 0.95f;
+  int k = 0;
+  int mode = GSDR_MODE_BATCHED;
+  int spec = 0;
+  int device = 0;
+};
+
+Options parse(int argc, char** argv) {
+  Options o;
+  int i = 1;
+  while (i < argc) {
+    std::string key = argv[i];
+    if (key.rfind("--", 0) == 0) {
+      if (i + 1 >= argc) usage();
+      std::string val = argv[i + 1];
+      if (key == "--mode") o.mode = (val == "plain") ? GSDR_MODE_PLAIN : (val == "batched" ? GSDR_MODE_BATCHED : -1);
+      else if (key == "--spec") o.spec = std::atoi(val.c_str());
+      else if (key == "--membership") o.membership = val;
+      else if (key == "--report") o.report = val;
+      else if (key == "--device") o.device = std::atoi(val.c_str());
+      else usage();
+      if (o.mode < 0) usage();
+      i += 2;
+      continue;
+    }
+    if (i + 1 >= argc) usage();
+    const char* k = argv[i];
+    if (k[0] == '-' && k[1] != 0) ++k;
+    try {
+      switch (k[0]) {
+        case 'i': o.in = argv[i + 1]; break;
+        case 'o': o.out = argv[i + 1]; break;
+        case 't': o.t = std::stof(argv[i + 1]); break;
+        case 'w': o.k = std::stoi(argv[i + 1]); break;
+        default: usage();
+      }
+    } catch (...) {
+      usage();
+    }
+    i += 2;
+  }
+  // func.cpp:31-52
+  if (o.t < 0.8 || o.t >= 1) {
+    std::cout << "similarity out of range" << std::endl << "0.8<=similarity<1" << std::endl;
+    usage();
+  }
+  if (o.k == 0) {
+    o.k = gsdr::auto_word_length(o.t);
+  } else if (o.k < 4 || o.k > 8) {
+    std::cout << "word length out of range" << std::endl << "4<=word length<=8" << std::endl;
+    usage();
+  }
+  return o;
+}
+
+int die(const char* what, int st) {
+  std::cerr << "cluster: " << what << ": " << gsdr_strerror(st) << ": " << gsdr_last_error() << std::endl;
+  return 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const auto t0 = std::chrono::steady_clock::now();
+  Options o = parse(argc, argv);
+  std::cout << "input file:\t" << o.in << std::endl;
+  std::cout << "output file:\t" << o.out << std::endl;
+  std::cout << "similarity:\t" << o.t << std::endl;
+  std::cout << "word length:\t" << o.k << std::endl;
+
+  gsdr_ctx* ctx = nullptr;
+  int st = gsdr_open(o.device, &ctx);
+  if (st) return die("open device", st);
+  char name[256];
+  gsdr_device_info(ctx, name, sizeof(name), nullptr, nullptr);
+  std::cout << "Using: " << name << std::endl;
+
+  gsdr_records* recs = nullptr;
+  if ((st = gsdr_records_read_fasta(o.in.c_str(), &recs))) return die("read", st);
+  const int64_t n = gsdr_records_count(recs);
+  std::vector<int64_t> perm((size_t)n);
+  if ((st = gsdr_records_sort(recs, perm.data()))) return die("sort", st);
+  int64_t first_len = 0, last_len = 0;
+  gsdr_records_get(recs, 0, nullptr, nullptr, nullptr, &first_len);
+  gsdr_records_get(recs, n - 1, nullptr, nullptr, nullptr, &last_len);
+  std::cout << "read file complete" << std::endl;
+  std::cout << "longest/shortest:\t" << first_len << "/" << last_len << std::endl;
+  std::cout << "reads count:\t" << n << std::endl;
+
+  gsdr_store* store = nullptr;
+  if ((st = gsdr_load(ctx, recs, o.k, &store))) return die("load", st);
+  const auto t1 = std::chrono::steady_clock::now();
+
+  gsdr_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.threshold = o.t;
+  p.word_length = o.k;
+  p.mode = o.mode;
+  p.spec_reps = o.spec;
+  std::vector<int32_t> cl((size_t)n);
+  gsdr_stats stats;
+  std::memset(&stats, 0, sizeof(stats));
+  if ((st = gsdr_cluster(ctx, store, &p, cl.data(), &stats))) return die("cluster", st);
+
+  int64_t reps = 0;
+  if ((st = gsdr_records_write_result(recs, cl.data(), o.out.c_str(), &reps))) return die("write", st);
+  if (!o.membership.empty() &&
+      (st = gsdr_records_write_membership(recs, perm.data(), cl.data(), o.membership.c_str())))
+    return die("membership", st);
+  const auto t2 = std::chrono::steady_clock::now();
+  const double wall = std::chrono::duration<double>(t2 - t0).count();
+  const double dev = stats.ms_total / 1e3;
+  std::cout << "cluster:\t" << reps << std::endl;
+  std::cout << "time consuming:\t" << (long long)(wall * 1000.0) << "ms" << std::endl;
+  if (!o.report.empty()) {
+    FILE* f = std::fopen(o.report.c_str(), "w");
+    if (f) {
+      std::fprintf(f,
+                   "{\"n_seqs\": %lld, \"n_bases\": %lld, \"reps\": %lld, \"rounds\": %lld, "
+                   "\"wall_s\": %.6f, \"load_s\": %.6f, \"cluster_s\": %.6f, "
+                   "\"clustered_mbp_per_s\": %.3f, \"device_mbp_per_s\": %.3f, "
+                   "\"pairs\": %lld, \"rej_length\": %lld, \"rej_comp\": %lld, \"rej_word\": %lld, "
+                   "\"aligned\": %lld, \"accepted\": %lld, \"rej_align\": %lld, "
+                   "\"cells_band\": %.0f, \"cells_exec\": %.0f, \"cells_full\": %.0f}\n",
+                   (long long)stats.n_seqs, (long long)stats.n_bases, (long long)reps, (long long)stats.rounds,
+                   wall, std::chrono::duration<double>(t1 - t0).count(), dev,
+                   (double)stats.n_bases / 1e6 / wall, (double)stats.n_bases / 1e6 / (dev > 0 ? dev : 1e-9),
+                   (long long)stats.pairs, (long long)stats.rej_length, (long long)stats.rej_comp,
+                   (long long)stats.rej_word, (long long)stats.aligned, (long long)stats.accepted,
+                   (long long)stats.rej_align, stats.cells_band, stats.cells_exec, stats.cells_full);
+      std::fclose(f);
+    }
+  }
+  gsdr_store_free(store);
+  gsdr_records_free(recs);
+  gsdr_close(ctx);
+  return 0;
+}

@@ -1,0 +1,355 @@
+// Store construction and the plain per-round driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "driver.hpp"
+#include "kernels_impl.hpp"
+
+namespace gsdr {
+
+#define HIP_TRY(expr, what)                          \
+  do {                                               \
+    hipError_t _e = (expr);                          \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+template <typename T>
+static int dev_alloc(gsdr_store& s, T** p, size_t count) {
+  size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) return fail(GSDR_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  s.allocs.push_back((void*)*p);
+  s.footprint += (int64_t)bytes;
+  return GSDR_OK;
+}
+
+int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
+  HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+  Prep p;
+  preprocess(sorted, k, host_threads(), p);
+  const int64_t n = p.n;
+  s.n = (int32_t)n;
+  s.k = k;
+  s.bases = (int64_t)sorted.data.size();
+  s.max_m = p.max_m;
+  s.raw_len = p.raw_len;
+  s.m = p.m;
+  int st;
+  if ((st = dev_alloc(s, &s.pk, (size_t)p.pk_off[n])) ||
+      (st = dev_alloc(s, &s.pkoff, (size_t)n + 1)) ||
+      (st = dev_alloc(s, &s.m_d, (size_t)n)) ||
+      (st = dev_alloc(s, &s.comp, (size_t)n)) ||
+      (st = dev_alloc(s, &s.kp, (size_t)p.kp_off[n])) ||
+      (st = dev_alloc(s, &s.kpoff, (size_t)n + 1)) ||
+      (st = dev_alloc(s, &s.bc, (size_t)n)) ||
+      (st = dev_alloc(s, &s.wc, (size_t)n)) ||
+      (st = dev_alloc(s, &s.cluster, (size_t)n)) ||
+      (st = dev_alloc(s, &s.cand[0], (size_t)n)) ||
+      (st = dev_alloc(s, &s.cand[1], (size_t)n)) ||
+      (st = dev_alloc(s, &s.items, (size_t)n * kNumShapes)) ||
+      (st = dev_alloc(s, &s.counters, (size_t)2 * kNumShapes)) ||
+      (st = dev_alloc(s, &s.bcount, (size_t)(n + 1023) / 1024 + 1)) ||
+      (st = dev_alloc(s, &s.st, (size_t)1)) ||
+      (st = dev_alloc(s, &s.stats, (size_t)ST_NUM)))
+    return st;
+  hipStream_t q = c->stream;
+  HIP_TRY(hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload pk");
+  HIP_TRY(hipMemcpyAsync(s.pkoff, p.pk_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, q), "upload pkoff");
+  HIP_TRY(hipMemcpyAsync(s.m_d, p.m.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, q), "upload m");
+  HIP_TRY(hipMemcpyAsync(s.comp, p.comp.data(), n * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload comp");
+  if (!p.kp.empty())
+    HIP_TRY(hipMemcpyAsync(s.kp, p.kp.data(), p.kp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload kp");
+  HIP_TRY(hipMemcpyAsync(s.kpoff, p.kp_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, q), "upload kpoff");
+  HIP_TRY(hipStreamSynchronize(q), "upload sync");
+  return GSDR_OK;
+}
+
+void store_release(gsdr_store& s) {
+  if (s.ctx) hipSetDevice(s.ctx->device);
+  for (void* p : s.allocs) hipFree(p);
+  s.allocs.clear();
+  s.footprint = 0;
+}
+
+// Cutoffs for threshold t (func.cpp:339-374), computed on the host with the
+// reference's float32 expressions and uploaded.
+int store_set_threshold(gsdr_store& s, float t) {
+  if (s.t_loaded == t) return GSDR_OK;
+  std::vector<int32_t> bc((size_t)s.n), wc((size_t)s.n);
+  for (int32_t i = 0; i < s.n; ++i) {
+    bc[i] = base_cutoff(s.raw_len[i], t);
+    wc[i] = word_cutoff(s.raw_len[i], t, s.k);
+  }
+  hipStream_t q = s.ctx->stream;
+  HIP_TRY(hipMemcpyAsync(s.bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice, q), "upload bc");
+  HIP_TRY(hipMemcpyAsync(s.wc, wc.data(), wc.size() * 4, hipMemcpyHostToDevice, q), "upload wc");
+  HIP_TRY(hipStreamSynchronize(q), "cutoff sync");
+  s.t_loaded = t;
+  return GSDR_OK;
+}
+
+// Option checks of func.cpp:31-51.
+int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out) {
+  const float t = p.threshold;
+  if (!(t >= 0.8f) || t >= 1.0f) return fail(GSDR_E_INVALID, "similarity out of range: 0.8<=similarity<1");
+  int k = p.word_length;
+  if (k == 0) k = auto_word_length(t);
+  if (k < 4 || k > 8) return fail(GSDR_E_INVALID, "word length out of range: 4<=word length<=8");
+  if (k != s.k) return fail(GSDR_E_INVALID, "store was built for another word length");
+  if (s.max_m > GSDR_MAX_GAPFREE)
+    return fail(GSDR_E_LIMIT, "a sequence exceeds 32767 gap-free bases (the representative limit)");
+  *k_out = k;
+  return GSDR_OK;
+}
+
+void stats_from_device(const unsigned long long* d, gsdr_stats* o) {
+  o->pairs += (int64_t)d[ST_PAIRS];
+  o->rej_length += (int64_t)d[ST_REJ_LEN];
+  o->rej_comp += (int64_t)d[ST_REJ_COMP];
+  o->rej_word += (int64_t)d[ST_REJ_WORD];
+  o->aligned += (int64_t)d[ST_ALIGNED];
+  o->accepted += (int64_t)d[ST_ACCEPT];
+  o->rej_align += (int64_t)d[ST_REJ_ALIGN];
+  o->cells_exec += (double)d[ST_CELLS_EXEC];
+  o->cells_band += (double)d[ST_CELLS_BAND];
+  o->cells_full += (double)d[ST_CELLS_FULL];
+  o->filter_bytes += (double)d[ST_FILTER_BYTES];
+  o->spec_wasted += (double)d[ST_SPEC_WASTE];
+}
+
+int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
+  if (p.mode == GSDR_MODE_BATCHED) {
+    gsdr_store* sp = &s;
+    return cluster_batched(c, 1, &sp, &p, &out, stats);
+  }
+  if (p.mode != GSDR_MODE_PLAIN) return fail(GSDR_E_INVALID, "unknown mode");
+  return cluster_plain(c, s, p, out, stats);
+}
+
+// Largest tile shape a representative of gap-free length n can need.
+static int max_shape_for(int n) {
+  const int ntw = (n >> 5) + 1;
+  for (int i = 0; i < kNumShapes; ++i)
+    if (kShapes[i].s * kShapes[i].w >= ntw) return i;
+  return kNumShapes - 1;
+}
+
+// Plain form: one representative per round. Per round the host launches the
+// filter, the alignment kernels of the tile shapes this representative can
+// need, and the compaction, then reads back the next representative (one
+// synchronisation per round, like the reference's per-round waits at
+// func.cpp:695-781 but without the host compaction loops).
+int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
+  int k = 0;
+  int st = check_params(s, p, &k);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+  st = store_set_threshold(s, p.threshold);
+  if (st) return st;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t q = c->stream;
+  const int n = s.n;
+  HIP_TRY(hipMemsetAsync(s.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
+  HIP_TRY(hipMemsetAsync(s.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
+  HIP_TRY(launch_init(s.cluster, s.cand[0], n, s.st, q), "init");
+  int64_t launches = 3;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  if (p.timing)
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  double ms_filter = 0, ms_align = 0;
+  int64_t filter_launches = 0, align_launches = 0;
+  int rep = 0, nc = n - 1, cur = 0;
+  int64_t rounds = 0;
+  const int cap_blocks = c->cu_count * 4;
+  const StoreView view = s.view();
+  while (rep >= 0) {
+    ++rounds;
+    if (nc > 0) {
+      if (p.timing) HIP_TRY(hipEventRecord(ev[0], q), "event");
+      FilterArgs fa;
+      fa.st = view;
+      fa.cand = s.cand[cur];
+      fa.ncp = &s.st->nc;
+      fa.repp = &s.st->rep;
+      fa.items = s.items;
+      fa.item_cap = n;
+      fa.min_shape = 0;
+      fa.bin_count = s.counters;
+      fa.stats = s.stats;
+      const int fgrid = std::min((nc + 255) / 256, cap_blocks);
+      HIP_TRY(launch_filter(k, fa, fgrid, q), "filter launch");
+      ++launches;
+      ++filter_launches;
+      if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
+      const int smax = max_shape_for(s.m[rep]);
+      for (int sh = 0; sh <= smax; ++sh) {
+        AlignArgs aa;
+        aa.st = view;
+        aa.items = s.items + (int64_t)sh * n;
+        aa.count = s.counters + sh;
+        aa.work = s.counters + kNumShapes + sh;
+        aa.cluster = s.cluster;
+        aa.result = nullptr;
+        aa.stats = s.stats;
+        const int segs = 64 / kShapes[sh].s;
+        const int waves = (nc + segs - 1) / segs;
+        const int grid = std::max(1, std::min((waves + 3) / 4, cap_blocks));
+        HIP_TRY(launch_align(sh, aa, grid, q), "align launch");
+        ++launches;
+        ++align_launches;
+      }
+      if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
+    }
+    HIP_TRY(launch_compact(s.cand[cur], s.st, s.cluster, s.bcount, s.cand[1 - cur], nc, s.counters,
+                           2 * kNumShapes, q),
+            "compact launch");
+    launches += 2;
+    HIP_TRY(hipMemcpyAsync(c->h_state, s.st, sizeof(RoundState), hipMemcpyDeviceToHost, q), "state copy");
+    HIP_TRY(hipStreamSynchronize(q), "round sync");
+    if (p.timing && nc > 0) {
+      float a = 0, b = 0;
+      HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
+      HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
+      ms_filter += a;
+      ms_align += b;
+    }
+    const int next = c->h_state->rep;
+    const int next_nc = c->h_state->nc;
+    // each round fixes one representative and shrinks the candidate list
+    if (next >= 0 && (next <= rep || next_nc >= nc || rounds > (int64_t)n))
+      return fail(GSDR_E_DEVICE, "round state did not advance");
+    rep = next;
+    nc = next_nc;
+    cur ^= 1;
+  }
+  HIP_TRY(hipMemcpyAsync(out, s.cluster, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, q), "cluster copy");
+  unsigned long long hst[ST_NUM];
+  HIP_TRY(hipMemcpyAsync(hst, s.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
+  HIP_TRY(hipStreamSynchronize(q), "final sync");
+  if (p.timing)
+    for (auto& e : ev) hipEventDestroy(e);
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats_from_device(hst, stats);
+    stats->n_seqs = n;
+    stats->n_bases = s.bases;
+    int64_t reps = 0;
+    for (int i = 0; i < n; ++i) reps += out[i] == i;
+    stats->reps = reps;
+    stats->rounds = rounds;
+    stats->launches = launches;
+    stats->ms_total = ms;
+    stats->ms_filter = ms_filter;
+    stats->ms_align = ms_align;
+    stats->filter_launches = filter_launches;
+    stats->align_launches = align_launches;
+  }
+  return GSDR_OK;
+}
+
+// Pair-level entry: LCS(text, query) > bc per pair, on the device.
+int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq, const int32_t* pairs,
+                const int32_t* bc, int npairs, int shape, int32_t* accept_out, double* cells_exec_out) {
+  HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+  Records r;
+  for (int i = 0; i < nseq; ++i) r.push("", 0, data + data_off[i], data_off[i + 1] - data_off[i]);
+  Prep p;
+  preprocess(r, 4, host_threads(), p);
+  for (int i = 0; i < npairs; ++i) {
+    const int t = pairs[2 * i], qq = pairs[2 * i + 1];
+    if (t < 0 || t >= nseq || qq < 0 || qq >= nseq) return fail(GSDR_E_INVALID, "pair index out of range");
+    if (p.m[t] > GSDR_MAX_GAPFREE) return fail(GSDR_E_LIMIT, "text longer than 32767 gap-free bases");
+    if (bc[i] < 0) return fail(GSDR_E_INVALID, "negative cutoff");
+  }
+  if (npairs == 0) {
+    if (cells_exec_out) *cells_exec_out = 0;
+    return GSDR_OK;
+  }
+  const int min_shape = shape < 0 ? 0 : std::min(shape, kNumShapes - 1);
+  // a private store: sequences, plus per-pair cutoffs indexed through a
+  // per-pair query copy (query slot = nseq + pair) so each pair has its bc.
+  gsdr_store s;
+  s.ctx = c;
+  const int64_t nslots = (int64_t)nseq + npairs;
+  std::vector<int64_t> pkoff((size_t)nslots + 1);
+  std::vector<int32_t> m((size_t)nslots), bcv((size_t)nslots, 0);
+  for (int i = 0; i < nseq; ++i) { pkoff[i] = p.pk_off[i]; m[i] = p.m[i]; }
+  for (int i = 0; i < npairs; ++i) {
+    const int qq = pairs[2 * i + 1];
+    pkoff[nseq + i] = p.pk_off[qq];
+    m[nseq + i] = p.m[qq];
+    bcv[nseq + i] = bc[i];
+  }
+  std::vector<std::vector<int2>> bins(kNumShapes);
+  std::vector<std::vector<int>> bin_pair(kNumShapes);
+  for (int i = 0; i < npairs; ++i) {
+    const int t = pairs[2 * i];
+    const int sh = pick_shape(p.m[t], m[nseq + i], bcv[nseq + i] + 1, min_shape);
+    bins[sh].push_back(make_int2(t, nseq + i));
+    bin_pair[sh].push_back(i);
+  }
+  int st;
+  int2* items = nullptr;
+  int32_t* res = nullptr;
+  int32_t* ctr = nullptr;
+  if ((st = dev_alloc(s, &s.pk, p.pk.size() / 2)) || (st = dev_alloc(s, &s.pkoff, (size_t)nslots + 1)) ||
+      (st = dev_alloc(s, &s.m_d, (size_t)nslots)) || (st = dev_alloc(s, &s.bc, (size_t)nslots)) ||
+      (st = dev_alloc(s, &items, (size_t)npairs)) || (st = dev_alloc(s, &res, (size_t)npairs)) ||
+      (st = dev_alloc(s, &ctr, (size_t)2 * kNumShapes)) || (st = dev_alloc(s, &s.stats, (size_t)ST_NUM))) {
+    store_release(s);
+    return st;
+  }
+  hipStream_t q = c->stream;
+  std::vector<int2> all;
+  std::vector<int32_t> counts(2 * kNumShapes, 0);
+  std::vector<int> first(kNumShapes, 0);
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    first[sh] = (int)all.size();
+    counts[sh] = (int)bins[sh].size();
+    all.insert(all.end(), bins[sh].begin(), bins[sh].end());
+  }
+  auto cleanup_fail = [&](hipError_t e, const char* what) { store_release(s); return hip_fail(e, what); };
+  hipError_t e;
+  if ((e = hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(s.pkoff, pkoff.data(), pkoff.size() * 8, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(s.m_d, m.data(), m.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(s.bc, bcv.data(), bcv.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(items, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(ctr, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemsetAsync(s.stats, 0, ST_NUM * 8, q)))
+    return cleanup_fail(e, "align_pairs upload");
+  const StoreView view = s.view();
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    if (counts[sh] == 0) continue;
+    AlignArgs aa;
+    aa.st = view;
+    aa.items = items + first[sh];
+    aa.count = ctr + sh;
+    aa.work = ctr + kNumShapes + sh;
+    aa.cluster = nullptr;
+    aa.result = res + first[sh];
+    aa.stats = s.stats;
+    const int segs = 64 / kShapes[sh].s;
+    const int waves = (counts[sh] + segs - 1) / segs;
+    const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * 4));
+    if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return cleanup_fail(e, "align launch");
+  }
+  std::vector<int32_t> got(all.size());
+  unsigned long long hst[ST_NUM];
+  if ((e = hipMemcpyAsync(got.data(), res, got.size() * 4, hipMemcpyDeviceToHost, q)) ||
+      (e = hipMemcpyAsync(hst, s.stats, sizeof(hst), hipMemcpyDeviceToHost, q)) ||
+      (e = hipStreamSynchronize(q)))
+    return cleanup_fail(e, "align_pairs download");
+  for (int sh = 0; sh < kNumShapes; ++sh)
+    for (size_t j = 0; j < bin_pair[sh].size(); ++j) accept_out[bin_pair[sh][j]] = got[first[sh] + j];
+  if (cells_exec_out) *cells_exec_out = (double)hst[ST_CELLS_EXEC];
+  store_release(s);
+  return GSDR_OK;
+}
+
+}  // namespace gsdr

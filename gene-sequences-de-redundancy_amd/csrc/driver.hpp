@@ -1,0 +1,81 @@
+// Host-side objects behind the opaque C ABI handles, and the drivers that
+// launch the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gsdr.h"
+#include "host.hpp"
+#include "kernels.hpp"
+#include "status.hpp"
+
+struct gsdr_records {
+  gsdr::Records r;
+};
+
+struct gsdr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int64_t hbm_bytes = 0;
+  char name[256] = {0};
+  gsdr::RoundState* h_state = nullptr;  // pinned host copy of the round state
+};
+
+struct gsdr_store {
+  gsdr_ctx* ctx = nullptr;
+  int32_t n = 0;
+  int k = 0;
+  int64_t bases = 0;
+  int32_t max_m = 0;
+  std::vector<int32_t> raw_len, m;  // host copies
+  // HBM-resident store
+  uint2* pk = nullptr;
+  int64_t* pkoff = nullptr;
+  int32_t* m_d = nullptr;
+  uint4* comp = nullptr;
+  uint32_t* kp = nullptr;
+  int64_t* kpoff = nullptr;
+  int32_t* bc = nullptr;
+  int32_t* wc = nullptr;
+  float t_loaded = -1.0f;
+  // per-call workspace (allocated once with the store)
+  int32_t* cluster = nullptr;
+  int32_t* cand[2] = {nullptr, nullptr};
+  int2* items = nullptr;            // kNumShapes lists of n pairs
+  int32_t* counters = nullptr;      // bin counts [kNumShapes] then work counters [kNumShapes]
+  int32_t* bcount = nullptr;        // compaction block counts
+  gsdr::RoundState* st = nullptr;
+  unsigned long long* stats = nullptr;
+  int64_t footprint = 0;
+  std::vector<void*> allocs;
+
+  gsdr::StoreView view() const {
+    gsdr::StoreView v;
+    v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
+    return v;
+  }
+};
+
+namespace gsdr {
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int host_threads();
+
+int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s);
+void store_release(gsdr_store& s);
+int store_set_threshold(gsdr_store& s, float t);
+int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out);
+
+int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
+int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
+int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
+                    int32_t* const* out, gsdr_stats* stats);
+int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
+                 int32_t* const* out, gsdr_stats* stats);
+int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq, const int32_t* pairs,
+                const int32_t* bc, int npairs, int shape, int32_t* accept_out, double* cells_exec_out);
+void stats_from_device(const unsigned long long* d, gsdr_stats* out);
+}  // namespace gsdr

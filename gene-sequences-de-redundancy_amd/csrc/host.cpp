@@ -1,0 +1,475 @@
+// Host-side loader, length order, preprocessing, generator and writer.
+// Reference semantics cited per function (files of JuZhenCS/gene-sequences-de-redundancy).
+#include "host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <thread>
+#include <unistd.h>
+
+#include "status.hpp"
+
+namespace gsdr {
+
+// ---------------------------------------------------------------- encoding
+static uint8_t code_of(unsigned char c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': case 'U': case 'u': return 3;
+    default: return 4;
+  }
+}
+struct CodeTable {
+  uint8_t v[256];
+  CodeTable() { for (int i = 0; i < 256; ++i) v[i] = code_of((unsigned char)i); }
+};
+static const CodeTable kTable;
+static inline uint8_t base_code(unsigned char c) { return kTable.v[c]; }
+
+// ------------------------------------------------------------------ reader
+// func.cpp:70-104. The first line of the file is the first record's name
+// (verbatim). Each later line that starts with '>' starts a new record;
+// every other line (blank lines included) is appended raw to the current
+// record's data, so a '\r' of CRLF files stays in the data. Where the
+// reference loops forever (no trailing newline, a header as the last line,
+// an empty file) this reader ends: the last partial line counts as a line
+// and a trailing header becomes a record with empty data.
+int read_fasta(const char* path, Records& out, std::string& err) {
+  out.clear();
+  int fd = ::open(path, O_RDONLY);
+  if (fd < 0) { err = std::string("cannot open input file: ") + path; return GSDR_E_IO; }
+  struct stat st;
+  if (fstat(fd, &st) != 0) { ::close(fd); err = "cannot stat input file"; return GSDR_E_IO; }
+  size_t size = (size_t)st.st_size;
+  if (size == 0) { ::close(fd); err = "input file is empty"; return GSDR_E_INPUT; }
+  const char* buf = (const char*)mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+  ::close(fd);
+  if (buf == MAP_FAILED) { err = "cannot map input file"; return GSDR_E_IO; }
+  madvise((void*)buf, size, MADV_SEQUENTIAL);
+
+  out.names.reserve(size / 64 + 16);
+  out.data.reserve(size);
+  size_t pos = 0;
+  bool first = true;
+  while (pos < size) {
+    const char* nl = (const char*)memchr(buf + pos, '\n', size - pos);
+    size_t end = nl ? (size_t)(nl - buf) : size;
+    const char* line = buf + pos;
+    size_t len = end - pos;
+    if (first || (len > 0 && line[0] == '>')) {
+      if (!first) out.data_off.push_back((int64_t)out.data.size());
+      out.names.insert(out.names.end(), line, line + len);
+      out.name_off.push_back((int64_t)out.names.size());
+      first = false;
+    } else {
+      out.data.insert(out.data.end(), line, line + len);
+    }
+    pos = end + 1;
+  }
+  out.data_off.push_back((int64_t)out.data.size());
+  munmap((void*)buf, size);
+  return GSDR_OK;
+}
+
+// ------------------------------------------------------------------- order
+// func.cpp:105-107: std::sort(reads, [](a, b) { return a.data.size() >
+// b.data.size(); }). libstdc++'s introsort makes the same moves for any
+// element type given the same comparison results, so sorting (length, index)
+// keys with the same comparator yields the reference's permutation,
+// including its (unstable) order of equal lengths.
+namespace {
+struct Key { int64_t len; int64_t idx; };
+}
+std::vector<int64_t> length_order(const Records& r) {
+  const int64_t n = r.count();
+  std::vector<Key> keys((size_t)n);
+  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{r.len(i), i};
+  std::sort(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; });
+  std::vector<int64_t> perm((size_t)n);
+  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = keys[(size_t)i].idx;
+  return perm;
+}
+
+Records permute(const Records& r, const std::vector<int64_t>& perm) {
+  Records o;
+  o.names.reserve(r.names.size());
+  o.data.reserve(r.data.size());
+  o.name_off.reserve(perm.size() + 1);
+  o.data_off.reserve(perm.size() + 1);
+  for (int64_t src : perm) {
+    o.names.insert(o.names.end(), r.names.begin() + r.name_off[src], r.names.begin() + r.name_off[src + 1]);
+    o.name_off.push_back((int64_t)o.names.size());
+    o.data.insert(o.data.end(), r.data.begin() + r.data_off[src], r.data.begin() + r.data_off[src + 1]);
+    o.data_off.push_back((int64_t)o.data.size());
+  }
+  return o;
+}
+
+// ------------------------------------------------------------------ writer
+// func.cpp:785-796: for s ascending with cluster[s] == s, "name\n" then the
+// raw data and "\n".
+int64_t write_result(const Records& sorted, const int32_t* cluster, const char* path) {
+  FILE* f = fopen(path, "wb");
+  if (!f) return -1;
+  std::vector<char> buf;
+  buf.reserve(1 << 22);
+  int64_t reps = 0;
+  const int64_t n = sorted.count();
+  for (int64_t s = 0; s < n; ++s) {
+    if (cluster[s] != (int32_t)s) continue;
+    ++reps;
+    buf.insert(buf.end(), sorted.names.begin() + sorted.name_off[s], sorted.names.begin() + sorted.name_off[s + 1]);
+    buf.push_back('\n');
+    buf.insert(buf.end(), sorted.data.begin() + sorted.data_off[s], sorted.data.begin() + sorted.data_off[s + 1]);
+    buf.push_back('\n');
+    if (buf.size() > (1u << 22)) {
+      if (fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { fclose(f); return -1; }
+      buf.clear();
+    }
+  }
+  if (!buf.empty() && fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { fclose(f); return -1; }
+  if (fclose(f) != 0) return -1;
+  return reps;
+}
+
+int write_fasta(const Records& r, const char* path, int width) {
+  FILE* f = fopen(path, "wb");
+  if (!f) return GSDR_E_IO;
+  std::vector<char> buf;
+  buf.reserve(1 << 22);
+  for (int64_t i = 0; i < r.count(); ++i) {
+    buf.insert(buf.end(), r.names.begin() + r.name_off[i], r.names.begin() + r.name_off[i + 1]);
+    buf.push_back('\n');
+    int64_t a = r.data_off[i], b = r.data_off[i + 1];
+    for (int64_t p = a; p < b; p += width) {
+      int64_t e = std::min<int64_t>(b, p + width);
+      buf.insert(buf.end(), r.data.begin() + p, r.data.begin() + e);
+      buf.push_back('\n');
+    }
+    if (buf.size() > (1u << 22)) {
+      if (fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { fclose(f); return GSDR_E_IO; }
+      buf.clear();
+    }
+  }
+  if (!buf.empty() && fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { fclose(f); return GSDR_E_IO; }
+  return fclose(f) == 0 ? GSDR_OK : GSDR_E_IO;
+}
+
+// ----------------------------------------------------------------- cutoffs
+// func.cpp:36-45: the float threshold is compared with double literals.
+int auto_word_length(float t) {
+  if (t < 0.88) return 4;
+  if (t < 0.94) return 5;
+  if (t < 0.97) return 6;
+  return 7;
+}
+// func.cpp:368-369: ceil((float)L * t) in float32.
+int base_cutoff(int32_t raw_len, float t) {
+  volatile float prod = (float)raw_len * t;
+  return (int)std::ceil((float)prod);
+}
+// func.cpp:353-366 (the offset factor is overwritten with 1 at :364).
+int word_cutoff(int32_t raw_len, float t, int k) {
+  int required = raw_len - k + 1;
+  volatile float one_minus = 1.0f - t;
+  volatile float prod = (float)raw_len * one_minus;
+  float ce = std::ceil((float)prod);
+  volatile float scaled = ce * (float)k;
+  int cutoff = (int)(float)scaled;
+  required -= cutoff;
+  if (required < 1) required = 1;
+  return required;
+}
+
+// ------------------------------------------------------------- preprocess
+// Builds, per sorted sequence: composition of codes 0..3 (func.cpp:207-232),
+// the gap-free sequence packed as two bit planes per 32 bases with bit j of
+// word i holding base 32i+j (func.cpp:375-446; lo plane = bit 0 of the code,
+// hi plane = bit 1), and the multiset of k-mers over raw windows that contain
+// no code-4 base (func.cpp:252-338) stored as distinct (word, count) entries.
+// Counts are kept modulo 2^16 like the reference's unsigned short counters.
+void preprocess(const Records& sorted, int k, int threads, Prep& p) {
+  const int64_t n = sorted.count();
+  p.n = (int32_t)n;
+  p.k = k;
+  p.raw_len.assign((size_t)n, 0);
+  p.m.assign((size_t)n, 0);
+  p.comp.assign((size_t)n * 4, 0);
+  p.pk_off.assign((size_t)n + 1, 0);
+  p.kp_off.assign((size_t)n + 1, 0);
+  // pass 1: lengths, gaps, composition, profile sizes need the data; do
+  // packing offsets from raw lengths (upper bound: words = m/32+1 <= L/32+1),
+  // but use the exact m so the store is compact.
+  if (threads < 1) threads = 1;
+  std::vector<int32_t> distinct((size_t)n, 0);
+  auto pass1 = [&](int64_t lo, int64_t hi) {
+    const uint32_t nwords = 1u << (2 * k);
+    std::vector<uint32_t> cnt(nwords, 0);
+    std::vector<uint16_t> touched;
+    for (int64_t s = lo; s < hi; ++s) {
+      const char* d = sorted.data.data() + sorted.data_off[s];
+      const int64_t L = sorted.len(s);
+      uint32_t c4[5] = {0, 0, 0, 0, 0};
+      for (int64_t i = 0; i < L; ++i) c4[base_code((unsigned char)d[i])]++;
+      p.raw_len[s] = (int32_t)L;
+      p.m[s] = (int32_t)(c4[0] + c4[1] + c4[2] + c4[3]);
+      for (int b = 0; b < 4; ++b) p.comp[s * 4 + b] = c4[b];
+      // distinct k-mers
+      touched.clear();
+      uint32_t word = 0, mask = nwords - 1;
+      int valid = 0;
+      for (int64_t i = 0; i < L; ++i) {
+        uint8_t c = base_code((unsigned char)d[i]);
+        if (c == 4) { valid = 0; word = 0; continue; }
+        word = ((word << 2) | c) & mask;
+        if (++valid >= k) {
+          if (cnt[word]++ == 0) touched.push_back((uint16_t)word);
+        }
+      }
+      distinct[s] = (int32_t)touched.size();
+      for (uint16_t w : touched) cnt[w] = 0;
+    }
+  };
+  auto run = [&](auto fn) {
+    std::vector<std::thread> th;
+    int64_t chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+      int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
+      if (lo >= hi) break;
+      th.emplace_back(fn, lo, hi);
+    }
+    for (auto& x : th) x.join();
+  };
+  run(pass1);
+  p.max_m = 0;
+  for (int64_t s = 0; s < n; ++s) {
+    p.pk_off[s + 1] = p.pk_off[s] + (p.m[s] / 32 + 1);
+    p.kp_off[s + 1] = p.kp_off[s] + distinct[s];
+    p.max_m = std::max(p.max_m, p.m[s]);
+  }
+  p.pk.assign((size_t)p.pk_off[n] * 2, 0u);
+  p.kp.assign((size_t)p.kp_off[n], 0u);
+  auto pass2 = [&](int64_t lo, int64_t hi) {
+    const uint32_t nwords = 1u << (2 * k);
+    std::vector<uint32_t> cnt(nwords, 0);
+    std::vector<uint16_t> touched;
+    for (int64_t s = lo; s < hi; ++s) {
+      const char* d = sorted.data.data() + sorted.data_off[s];
+      const int64_t L = sorted.len(s);
+      uint32_t* w2 = p.pk.data() + p.pk_off[s] * 2;
+      int64_t j = 0;
+      for (int64_t i = 0; i < L; ++i) {
+        uint8_t c = base_code((unsigned char)d[i]);
+        if (c == 4) continue;
+        w2[(j >> 5) * 2 + 0] |= (uint32_t)(c & 1) << (j & 31);
+        w2[(j >> 5) * 2 + 1] |= (uint32_t)(c >> 1) << (j & 31);
+        ++j;
+      }
+      touched.clear();
+      uint32_t word = 0, mask = nwords - 1;
+      int valid = 0;
+      for (int64_t i = 0; i < L; ++i) {
+        uint8_t c = base_code((unsigned char)d[i]);
+        if (c == 4) { valid = 0; word = 0; continue; }
+        word = ((word << 2) | c) & mask;
+        if (++valid >= k) {
+          if (cnt[word]++ == 0) touched.push_back((uint16_t)word);
+        }
+      }
+      uint32_t* kp = p.kp.data() + p.kp_off[s];
+      for (size_t e = 0; e < touched.size(); ++e) {
+        uint16_t w = touched[e];
+        kp[e] = (uint32_t)w | ((cnt[w] & 0xFFFFu) << 16);
+        cnt[w] = 0;
+      }
+    }
+  };
+  run(pass2);
+}
+
+// -------------------------------------------------------------- generator
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+uint64_t job_seed(uint64_t base, uint64_t j) {
+  uint64_t x = base ^ (0xd1b54a32d192ed03ull * (j + 1));
+  return splitmix64(x);
+}
+
+namespace {
+// xoshiro256** seeded from splitmix64.
+struct Rng {
+  uint64_t s[4];
+  explicit Rng(uint64_t seed) { uint64_t x = seed; for (auto& v : s) v = splitmix64(x); }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uni() { return (double)(next() >> 11) * 0x1.0p-53; }
+  uint32_t below(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
+  int64_t range(int64_t lo, int64_t hi) { return lo + (int64_t)(uni() * (double)(hi - lo + 1)); }
+  double normal() {
+    double u1 = uni(), u2 = uni();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+const char kACGT[4] = {'A', 'C', 'G', 'T'};
+
+void random_seq(Rng& g, int64_t L, std::string& out) {
+  out.resize((size_t)L);
+  for (int64_t i = 0; i < L; ++i) out[(size_t)i] = kACGT[g.below(4)];
+}
+// Per parent base: deleted with p_del; otherwise substituted with p_sub (to
+// one of the three other bases); then a uniform base is inserted after it
+// with p_ins.
+void mutate(Rng& g, const std::string& parent, double p_sub, double p_ins, double p_del, std::string& out) {
+  out.clear();
+  out.reserve(parent.size() + parent.size() / 8 + 8);
+  for (char b : parent) {
+    if (g.uni() >= p_del) {
+      if (g.uni() < p_sub) {
+        int c = 0;
+        while (kACGT[c] != b) ++c;
+        out.push_back(kACGT[(c + 1 + g.below(3)) & 3]);
+      } else {
+        out.push_back(b);
+      }
+    }
+    if (g.uni() < p_ins) out.push_back(kACGT[g.below(4)]);
+  }
+}
+void shuffle_records(Rng& g, std::vector<std::string>& names, std::vector<std::string>& seqs) {
+  for (size_t i = seqs.size(); i > 1; --i) {
+    size_t j = (size_t)g.below((uint32_t)i);
+    std::swap(seqs[i - 1], seqs[j]);
+    std::swap(names[i - 1], names[j]);
+  }
+}
+void to_records(const std::vector<std::string>& names, const std::vector<std::string>& seqs, Records& out) {
+  out.clear();
+  for (size_t i = 0; i < seqs.size(); ++i)
+    out.push(names[i].data(), (int64_t)names[i].size(), seqs[i].data(), (int64_t)seqs[i].size());
+}
+int64_t lognormal_len(Rng& g, double median, double sigma, int64_t lo, int64_t hi) {
+  double v = std::exp(std::log(median) + sigma * g.normal());
+  int64_t L = (int64_t)std::llround(v);
+  return std::min(hi, std::max(lo, L));
+}
+}  // namespace
+
+void generate_family_job(uint64_t seed, int64_t n, double median, Records& out) {
+  Rng g(seed);
+  std::vector<std::string> names, seqs;
+  names.reserve((size_t)n); seqs.reserve((size_t)n);
+  std::string parent, child;
+  int64_t parents = (n + 9) / 10;
+  for (int64_t p = 0; p < parents && (int64_t)seqs.size() < n; ++p) {
+    int64_t L = lognormal_len(g, median, 0.5, 150, 6000);
+    random_seq(g, L, parent);
+    for (int c = 0; c < 10 && (int64_t)seqs.size() < n; ++c) {
+      double d = 0.12 * g.uni();
+      mutate(g, parent, 0.8 * d, 0.1 * d, 0.1 * d, child);
+      names.push_back(">f" + std::to_string(p) + "_c" + std::to_string(c));
+      seqs.push_back(child);
+    }
+  }
+  shuffle_records(g, names, seqs);
+  to_records(names, seqs, out);
+}
+
+void generate(const GenSpec& spec, Records& out) {
+  Rng g(spec.seed);
+  std::vector<std::string> names, seqs;
+  std::string parent, child;
+  if (spec.config == 1) {
+    // 500 parents, L ~ U{300..1500}; each parent plus 19 children
+    // (sub 0.02, ins 0.005, del 0.005).
+    int64_t total = spec.n_override > 0 ? spec.n_override : 10000;
+    int64_t parents = (total + 19) / 20;
+    for (int64_t p = 0; p < parents; ++p) {
+      int64_t L = g.range(300, 1500);
+      random_seq(g, L, parent);
+      for (int c = 0; c < 20 && (int64_t)seqs.size() < total; ++c) {
+        if (c == 0) child = parent;
+        else mutate(g, parent, 0.02, 0.005, 0.005, child);
+        names.push_back(">p" + std::to_string(p) + "_" + std::to_string(c));
+        seqs.push_back(child);
+      }
+    }
+    shuffle_records(g, names, seqs);
+    to_records(names, seqs, out);
+    return;
+  }
+  if (spec.config == 2) {
+    int64_t total = spec.n_override > 0 ? spec.n_override : 200000;
+    double median = spec.median > 0 ? spec.median : 1200.0;
+    generate_family_job(spec.seed, total, median, out);
+    return;
+  }
+  if (spec.config == 4) {
+    // Pareto(1.5, min 100) lengths clipped at 30000; 30% singletons, 70% in
+    // families of Geometric(1/8) size whose members are the (unemitted) root
+    // mutated at d ~ U[0, 0.06] (80% substitutions, 20% indels); 5% of all
+    // sequences get a tandem repeat (unit 1-6, run 50-500) written over a
+    // window at a uniform position.
+    int64_t total = spec.n_override > 0 ? spec.n_override : 2000000;
+    int64_t singles = (int64_t)std::llround(0.3 * (double)total);
+    auto pareto = [&]() {
+      double u = g.uni();
+      if (u < 1e-300) u = 1e-300;
+      double v = 100.0 * std::pow(u, -1.0 / 1.5);
+      int64_t L = v >= 30000.0 ? 30000 : (int64_t)std::floor(v);
+      return std::max<int64_t>(100, L);
+    };
+    names.reserve((size_t)total); seqs.reserve((size_t)total);
+    for (int64_t i = 0; i < singles; ++i) {
+      random_seq(g, pareto(), child);
+      names.push_back(">s" + std::to_string(i));
+      seqs.push_back(child);
+    }
+    int64_t fam = 0;
+    while ((int64_t)seqs.size() < total) {
+      int64_t size = 1;
+      while (g.uni() >= 1.0 / 8.0) ++size;
+      random_seq(g, pareto(), parent);
+      for (int64_t c = 0; c < size && (int64_t)seqs.size() < total; ++c) {
+        double d = 0.06 * g.uni();
+        mutate(g, parent, 0.8 * d, 0.1 * d, 0.1 * d, child);
+        names.push_back(">g" + std::to_string(fam) + "_" + std::to_string(c));
+        seqs.push_back(child);
+      }
+      ++fam;
+    }
+    for (auto& s : seqs) {
+      if (g.uni() >= 0.05) continue;
+      int unit_len = (int)g.range(1, 6);
+      char unit[6];
+      for (int u = 0; u < unit_len; ++u) unit[u] = kACGT[g.below(4)];
+      int64_t run = std::min<int64_t>((int64_t)s.size(), g.range(50, 500));
+      int64_t at = g.range(0, (int64_t)s.size() - run);
+      for (int64_t x = 0; x < run; ++x) s[(size_t)(at + x)] = unit[x % unit_len];
+    }
+    shuffle_records(g, names, seqs);
+    to_records(names, seqs, out);
+    return;
+  }
+  // config 3 jobs and config 5 requests are generated per job through
+  // generate_family_job with their own seeds and medians.
+  int64_t total = spec.n_override > 0 ? spec.n_override : 10000;
+  generate_family_job(spec.seed, total, spec.median > 0 ? spec.median : 1200.0, out);
+}
+
+}  // namespace gsdr

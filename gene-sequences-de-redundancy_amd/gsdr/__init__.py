@@ -1,0 +1,331 @@
+"""ctypes binding of libgsdr.so, the MI355X de-redundancy engine (include/gsdr.h).
+
+The library is built in-tree (``gene-sequences-de-redundancy_amd/build``) by
+``__graft_entry__.build()`` or ``make``. Loading fails loudly when it is
+missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD_DIR = os.path.join(PKG_DIR, "build")
+LIB_PATH = os.path.join(BUILD_DIR, "libgsdr.so")
+CLI_PATH = os.path.join(BUILD_DIR, "cluster")
+
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+MAX_GAPFREE = 32767
+
+OK, E_INVALID, E_IO, E_INPUT, E_DEVICE, E_NOMEM, E_LIMIT, E_BUSY, E_NOTFOUND = range(9)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("threshold", C.c_float),
+        ("word_length", C.c_int32),
+        ("mode", C.c_int32),
+        ("spec_reps", C.c_int32),
+        ("timing", C.c_int32),
+        ("tune", C.c_int32),
+        ("reserved", C.c_int32 * 2),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_seqs", C.c_int64),
+        ("n_bases", C.c_int64),
+        ("reps", C.c_int64),
+        ("rounds", C.c_int64),
+        ("launches", C.c_int64),
+        ("pairs", C.c_int64),
+        ("rej_length", C.c_int64),
+        ("rej_comp", C.c_int64),
+        ("rej_word", C.c_int64),
+        ("aligned", C.c_int64),
+        ("rej_align", C.c_int64),
+        ("accepted", C.c_int64),
+        ("cells_band", C.c_double),
+        ("cells_exec", C.c_double),
+        ("cells_full", C.c_double),
+        ("ms_total", C.c_double),
+        ("ms_filter", C.c_double),
+        ("ms_align", C.c_double),
+        ("filter_launches", C.c_int64),
+        ("align_launches", C.c_int64),
+        ("filter_bytes", C.c_double),
+        ("spec_wasted", C.c_double),
+        ("reserved", C.c_double * 4),
+    ]
+
+    def as_dict(self):
+        return {name: (getattr(self, name) if name != "reserved" else None)
+                for name, _ in self._fields_ if name != "reserved"}
+
+
+# every entry point of include/gsdr.h: (name, restype, argtypes)
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+SIGNATURES = [
+    ("gsdr_records_read_fasta", C.c_int, [C.c_char_p, _PP]),
+    ("gsdr_records_from_buffers", C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.c_char_p,
+                                            C.POINTER(C.c_int64), C.c_int64, _PP]),
+    ("gsdr_records_generate", C.c_int, [C.c_int32, C.c_uint64, C.c_int64, C.c_double, _PP]),
+    ("gsdr_records_write_fasta", C.c_int, [_P, C.c_char_p, C.c_int32]),
+    ("gsdr_records_sort", C.c_int, [_P, C.POINTER(C.c_int64)]),
+    ("gsdr_records_count", C.c_int64, [_P]),
+    ("gsdr_records_bases", C.c_int64, [_P]),
+    ("gsdr_records_get", C.c_int, [_P, C.c_int64, C.POINTER(C.c_char_p), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    ("gsdr_records_write_result", C.c_int, [_P, C.POINTER(C.c_int32), C.c_char_p, C.POINTER(C.c_int64)]),
+    ("gsdr_records_write_membership", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_char_p]),
+    ("gsdr_records_free", None, [_P]),
+    ("gsdr_open", C.c_int, [C.c_int32, _PP]),
+    ("gsdr_close", None, [_P]),
+    ("gsdr_device_info", C.c_int, [_P, C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    ("gsdr_load", C.c_int, [_P, _P, C.c_int32, _PP]),
+    ("gsdr_store_free", None, [_P]),
+    ("gsdr_store_footprint", C.c_int64, [_P]),
+    ("gsdr_cluster", C.c_int, [_P, _P, C.POINTER(Params), C.POINTER(C.c_int32), C.POINTER(Stats)]),
+    ("gsdr_cluster_jobs", C.c_int, [_P, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(Params),
+                                    C.POINTER(C.POINTER(C.c_int32)), C.POINTER(Stats)]),
+    ("gsdr_align_pairs", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]),
+    ("gsdr_serve_open", C.c_int, [_P, C.c_int64, C.c_int32, _PP]),
+    ("gsdr_serve_submit", C.c_int, [_P, _P, C.POINTER(Params), C.POINTER(C.c_int64)]),
+    ("gsdr_serve_step", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("gsdr_serve_poll", C.c_int, [_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(Stats), C.POINTER(C.c_int32)]),
+    ("gsdr_serve_close", None, [_P]),
+    ("gsdr_strerror", C.c_char_p, [C.c_int32]),
+    ("gsdr_last_error", C.c_char_p, []),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libgsdr.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libgsdr.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class GsdrError(RuntimeError):
+    def __init__(self, status, what):
+        L = lib()
+        msg = L.gsdr_last_error().decode(errors="replace")
+        super().__init__(f"{what}: {L.gsdr_strerror(status).decode()} ({status}): {msg}")
+        self.status = status
+
+
+def _check(st, what):
+    if st != OK:
+        raise GsdrError(st, what)
+
+
+def _i64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Records:
+    """FASTA records (names + raw data) owned by the library."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @staticmethod
+    def read_fasta(path):
+        h = C.c_void_p()
+        _check(lib().gsdr_records_read_fasta(os.fsencode(path), C.byref(h)), "read_fasta")
+        return Records(h)
+
+    @staticmethod
+    def generate(config, seed, n=0, median=0.0):
+        h = C.c_void_p()
+        _check(lib().gsdr_records_generate(config, seed, n, median, C.byref(h)), "generate")
+        return Records(h)
+
+    @staticmethod
+    def from_lists(datas, names=None):
+        datas = [d if isinstance(d, bytes) else d.encode() for d in datas]
+        if names is None:
+            names = [b">s%d" % i for i in range(len(datas))]
+        names = [n if isinstance(n, bytes) else n.encode() for n in names]
+        doff = np.zeros(len(datas) + 1, dtype=np.int64)
+        noff = np.zeros(len(names) + 1, dtype=np.int64)
+        doff[1:] = np.cumsum([len(d) for d in datas])
+        noff[1:] = np.cumsum([len(n) for n in names])
+        h = C.c_void_p()
+        _check(lib().gsdr_records_from_buffers(b"".join(names), _i64p(noff), b"".join(datas), _i64p(doff),
+                                               len(datas), C.byref(h)), "from_buffers")
+        return Records(h)
+
+    def __len__(self):
+        return lib().gsdr_records_count(self.h)
+
+    @property
+    def bases(self):
+        return lib().gsdr_records_bases(self.h)
+
+    def get(self, i):
+        nm = C.c_char_p()
+        nl = C.c_int64()
+        dp = C.c_void_p()
+        dl = C.c_int64()
+        _check(lib().gsdr_records_get(self.h, i, C.byref(nm), C.byref(nl), C.byref(dp), C.byref(dl)), "get")
+        name = C.string_at(nm, nl.value) if nl.value else b""
+        data = C.string_at(dp, dl.value) if dl.value else b""
+        return name, data
+
+    def sort(self):
+        perm = np.zeros(len(self), dtype=np.int64)
+        _check(lib().gsdr_records_sort(self.h, _i64p(perm)), "sort")
+        return perm
+
+    def write_fasta(self, path, width=80):
+        _check(lib().gsdr_records_write_fasta(self.h, os.fsencode(path), width), "write_fasta")
+
+    def write_result(self, cluster, path):
+        reps = C.c_int64()
+        cl = np.ascontiguousarray(cluster, dtype=np.int32)
+        _check(lib().gsdr_records_write_result(self.h, _i32p(cl), os.fsencode(path), C.byref(reps)), "write_result")
+        return reps.value
+
+    def write_membership(self, perm, cluster, path):
+        cl = np.ascontiguousarray(cluster, dtype=np.int32)
+        pp = np.ascontiguousarray(perm, dtype=np.int64)
+        _check(lib().gsdr_records_write_membership(self.h, _i64p(pp), _i32p(cl), os.fsencode(path)), "membership")
+
+    def free(self):
+        if self.h:
+            lib().gsdr_records_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def auto_word_length(t):
+    """func.cpp:36-45 (float threshold against double literals)."""
+    t = float(np.float32(t))
+    return 4 if t < 0.88 else 5 if t < 0.94 else 6 if t < 0.97 else 7
+
+
+class Engine:
+    """A device context on one GPU."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().gsdr_open(device, C.byref(self.h)), "open")
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int32()
+        hbm = C.c_int64()
+        _check(lib().gsdr_device_info(self.h, name, 256, C.byref(cu), C.byref(hbm)), "device_info")
+        return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": hbm.value}
+
+    def load(self, sorted_records, word_length):
+        h = C.c_void_p()
+        _check(lib().gsdr_load(self.h, sorted_records.h, word_length, C.byref(h)), "load")
+        return Store(self, h, len(sorted_records))
+
+    def align_pairs(self, datas, pairs, bc, shape=-1):
+        """LCS(text, query) > bc per (text, query) pair, decided on the device."""
+        datas = [d if isinstance(d, bytes) else d.encode() for d in datas]
+        doff = np.zeros(len(datas) + 1, dtype=np.int64)
+        doff[1:] = np.cumsum([len(d) for d in datas])
+        pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        bcs = np.ascontiguousarray(np.asarray(bc, dtype=np.int32))
+        out = np.zeros(len(pr), dtype=np.int32)
+        cells = C.c_double()
+        _check(lib().gsdr_align_pairs(self.h, b"".join(datas), _i64p(doff), len(datas), _i32p(pr), _i32p(bcs),
+                                      len(pr), shape, _i32p(out), C.byref(cells)), "align_pairs")
+        return out.astype(bool), cells.value
+
+    def cluster_jobs(self, stores, thresholds, mode=MODE_BATCHED, spec_reps=0):
+        n = len(stores)
+        params = (Params * n)()
+        for i, t in enumerate(thresholds):
+            params[i].threshold = t
+            params[i].word_length = stores[i].k if hasattr(stores[i], "k") else 0
+            params[i].mode = mode
+            params[i].spec_reps = spec_reps
+        outs = [np.zeros(s.n, dtype=np.int32) for s in stores]
+        handles = (C.c_void_p * n)(*[s.h.value for s in stores])
+        ptrs = (C.POINTER(C.c_int32) * n)(*[_i32p(o) for o in outs])
+        st = Stats()
+        _check(lib().gsdr_cluster_jobs(self.h, n, handles, params, ptrs, C.byref(st)), "cluster_jobs")
+        return outs, st
+
+    def close(self):
+        if self.h:
+            lib().gsdr_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Store:
+    """Sequences resident in HBM for one word length."""
+
+    def __init__(self, engine, handle, n):
+        self.engine = engine
+        self.h = handle
+        self.n = n
+        self.k = None
+
+    @property
+    def footprint(self):
+        return lib().gsdr_store_footprint(self.h)
+
+    def cluster(self, threshold, word_length=0, mode=MODE_PLAIN, spec_reps=0, timing=False, tune=False):
+        p = Params()
+        p.threshold = threshold
+        p.word_length = word_length
+        p.mode = mode
+        p.spec_reps = spec_reps
+        p.timing = 1 if timing else 0
+        p.tune = 1 if tune else 0
+        out = np.zeros(self.n, dtype=np.int32)
+        st = Stats()
+        _check(lib().gsdr_cluster(self.engine.h, self.h, C.byref(p), _i32p(out), C.byref(st)), "cluster")
+        return out, st
+
+    def free(self):
+        if self.h:
+            lib().gsdr_store_free(self.h)
+            self.h = None
+
+
+def cluster_records(engine, sorted_records, threshold, word_length=0, mode=MODE_PLAIN, **kw):
+    """Loads sorted records, clusters them and frees the store."""
+    k = word_length or auto_word_length(threshold)
+    store = engine.load(sorted_records, k)
+    store.k = k
+    try:
+        return store.cluster(threshold, k, mode=mode, **kw)
+    finally:
+        store.free()

@@ -1,0 +1,168 @@
+/*
+ * gsdr.h — C ABI of the MI355X (gfx950) exact greedy DNA de-redundancy engine.
+ *
+ * The reference program (JuZhenCS/gene-sequences-de-redundancy, root variant)
+ * has no plugin or FFI layer; its path sits behind the stage API of func.h and
+ * the `cluster` command line. Each entry point below names the reference
+ * interface it replaces. Conventions:
+ *   - every call returns an int status (GSDR_OK == 0); nothing throws;
+ *   - host buffers are owned by the caller, device memory by the context;
+ *   - one context per host thread; a context is not re-entrant;
+ *   - calls block, except the serving submit/step/poll family.
+ */
+#ifndef GSDR_H
+#define GSDR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GSDR_OK 0
+#define GSDR_E_INVALID 1   /* bad argument or option */
+#define GSDR_E_IO 2        /* file could not be read or written */
+#define GSDR_E_INPUT 3     /* malformed or unsupported input */
+#define GSDR_E_DEVICE 4    /* HIP runtime error */
+#define GSDR_E_NOMEM 5     /* allocation failed */
+#define GSDR_E_LIMIT 6     /* an implementation size limit was exceeded */
+#define GSDR_E_BUSY 7      /* serving: no slot or memory free right now */
+#define GSDR_E_NOTFOUND 8  /* serving: unknown job id */
+
+/* Longest gap-free sequence the engine aligns as a representative: 32,767
+ * bases, the same bound as the reference's local array (func.cpp:752-754). */
+#define GSDR_MAX_GAPFREE 32767
+
+typedef struct gsdr_ctx gsdr_ctx;
+typedef struct gsdr_records gsdr_records;
+typedef struct gsdr_store gsdr_store;
+typedef struct gsdr_server gsdr_server;
+
+enum gsdr_mode {
+  GSDR_MODE_PLAIN = 0,   /* one representative per round, kernels launched per round */
+  GSDR_MODE_BATCHED = 1  /* persistent rounds, speculative representatives, no host sync per round */
+};
+
+typedef struct gsdr_params {
+  float threshold;      /* t, parsed as float like std::stof (func.cpp:21); 0.8 <= t < 1 */
+  int32_t word_length;  /* k; 0 = from t (func.cpp:36-45), else 4..8 */
+  int32_t mode;         /* enum gsdr_mode */
+  int32_t spec_reps;    /* batched: representatives evaluated per speculative step (0 = default) */
+  int32_t timing;       /* 1 = time every filter/align launch with HIP events */
+  int32_t tune;         /* 1 = use the autotuned align shapes when available */
+  int32_t reserved[2];
+} gsdr_params;
+
+typedef struct gsdr_stats {
+  int64_t n_seqs;       /* records clustered */
+  int64_t n_bases;      /* raw bases (sum of record data lengths) */
+  int64_t reps;         /* representatives written */
+  int64_t rounds;       /* host-visible rounds (plain) or persistent steps (batched) */
+  int64_t launches;     /* kernel launches issued */
+  int64_t pairs;        /* (representative, candidate) pairs considered */
+  int64_t rej_length;   /* rejected: min(gap-free lengths) < ceil(L*t)+1 (added, sound) */
+  int64_t rej_comp;     /* rejected by the composition filter, incl. the sound tightening */
+  int64_t rej_word;     /* rejected by the shared-word filter (func.cpp:544-571) */
+  int64_t aligned;      /* pairs that reached the alignment */
+  int64_t rej_align;    /* aligned and rejected */
+  int64_t accepted;     /* aligned and absorbed into the representative's cluster */
+  double cells_band;    /* sum of exact LCS decision-band cells over aligned pairs */
+  double cells_exec;    /* bit-cells the alignment kernels actually computed */
+  double cells_full;    /* sum of n*m over aligned pairs (the reference computes all) */
+  double ms_total;      /* wall time of the clustering call */
+  double ms_filter;     /* filter kernel time (timing == 1) */
+  double ms_align;      /* align kernel time (timing == 1) */
+  int64_t filter_launches;
+  int64_t align_launches;
+  double filter_bytes;  /* algorithmic bytes read+written by the filter kernels */
+  double spec_wasted;   /* batched: pair evaluations discarded by speculation */
+  double reserved[4];
+} gsdr_stats;
+
+/* ---- records (host side): replaces readFile/saveFile (func.cpp:70-112,
+ *      785-799) and the Read vector (func.h:17-20) ---- */
+
+/* Reads FASTA with the reference's record split (func.cpp:70-104). */
+int gsdr_records_read_fasta(const char *path, gsdr_records **out);
+/* Builds records from caller buffers (offsets have count+1 entries). */
+int gsdr_records_from_buffers(const char *names, const int64_t *name_off, const char *data,
+                              const int64_t *data_off, int64_t count, gsdr_records **out);
+/* Synthetic mutated-family workloads (config 1..4; seed; n > 0 overrides
+ * the record count; median > 0 overrides the length median). */
+int gsdr_records_generate(int32_t config, uint64_t seed, int64_t n, double median,
+                          gsdr_records **out);
+int gsdr_records_write_fasta(const gsdr_records *r, const char *path, int32_t line_width);
+/* Reorders in place into the reference's length order (func.cpp:105-107);
+ * perm_out (may be NULL, count entries) gets the original index per slot. */
+int gsdr_records_sort(gsdr_records *r, int64_t *perm_out);
+int64_t gsdr_records_count(const gsdr_records *r);
+int64_t gsdr_records_bases(const gsdr_records *r);
+int gsdr_records_get(const gsdr_records *r, int64_t i, const char **name, int64_t *name_len,
+                     const char **data, int64_t *data_len);
+/* Writes representatives (cluster[s] == s) like saveFile (func.cpp:785-796). */
+int gsdr_records_write_result(const gsdr_records *sorted, const int32_t *cluster, const char *path,
+                              int64_t *reps_out);
+/* Writes one line per record: sorted index, original index (perm, may be
+ * NULL), representative's sorted index, name. */
+int gsdr_records_write_membership(const gsdr_records *sorted, const int64_t *perm,
+                                  const int32_t *cluster, const char *path);
+void gsdr_records_free(gsdr_records *r);
+
+/* ---- device context: replaces selectDevice (func.cpp:58-69) ---- */
+int gsdr_open(int32_t device, gsdr_ctx **ctx);
+void gsdr_close(gsdr_ctx *ctx);
+int gsdr_device_info(gsdr_ctx *ctx, char *name, int32_t name_len, int32_t *cu_count,
+                     int64_t *hbm_bytes);
+
+/* ---- sequence store in HBM: replaces copyData and the eight preprocessing
+ *      stages (func.cpp:113-446; func.h:52-60). `sorted` must already be in
+ *      the reference's length order. ---- */
+int gsdr_load(gsdr_ctx *ctx, const gsdr_records *sorted, int32_t word_length, gsdr_store **store);
+void gsdr_store_free(gsdr_store *store);
+int64_t gsdr_store_footprint(const gsdr_store *store);
+
+/* ---- clustering: replaces clustering(Option&, Data&, Bench&)
+ *      (func.cpp:677-784; func.h:61). cluster_out: caller-owned int32[n];
+ *      cluster_out[s] == s marks a representative, otherwise it holds the
+ *      representative's sorted index. ---- */
+int gsdr_cluster(gsdr_ctx *ctx, gsdr_store *store, const gsdr_params *params,
+                 int32_t *cluster_out, gsdr_stats *stats);
+
+/* Many independent jobs in shared persistent launches (no reference
+ * counterpart; each job's result equals gsdr_cluster on that job alone). */
+int gsdr_cluster_jobs(gsdr_ctx *ctx, int32_t njobs, gsdr_store *const *stores,
+                      const gsdr_params *params, int32_t *const *cluster_out, gsdr_stats *stats);
+
+/* ---- pair-level entry (diagnostics and parity tests): decides
+ *      LCS(text, query) > bc on the device, the test of kernel_dynamic
+ *      (func.cpp:572-676). Sequences are given as raw bytes like FASTA data;
+ *      pairs holds npairs (text index, query index) pairs. shape < 0 picks the
+ *      tile shape per pair; otherwise it forces shape index `shape`. ---- */
+int gsdr_align_pairs(gsdr_ctx *ctx, const char *data, const int64_t *data_off, int32_t nseq,
+                     const int32_t *pairs, const int32_t *bc, int32_t npairs, int32_t shape,
+                     int32_t *accept_out, double *cells_exec_out);
+
+/* ---- serving: continuous batching of clustering jobs ---- */
+int gsdr_serve_open(gsdr_ctx *ctx, int64_t hbm_budget, int32_t max_slots, gsdr_server **srv);
+/* Admits the job if its footprint fits the budget and a slot is free;
+ * otherwise returns GSDR_E_BUSY and the caller retries later. */
+int gsdr_serve_submit(gsdr_server *srv, const gsdr_records *sorted, const gsdr_params *params,
+                      int64_t *job_id);
+/* Advances every admitted job by one persistent step. Returns the number
+ * of jobs still running in *running. */
+int gsdr_serve_step(gsdr_server *srv, int32_t *running);
+/* *done = 1 when finished; then cluster_out (n entries) and stats are filled
+ * and the job's device memory is released. */
+int gsdr_serve_poll(gsdr_server *srv, int64_t job_id, int32_t *cluster_out, gsdr_stats *stats,
+                    int32_t *done);
+void gsdr_serve_close(gsdr_server *srv);
+
+const char *gsdr_strerror(int32_t status);
+/* Message of the last failing call on this thread. */
+const char *gsdr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSDR_H */

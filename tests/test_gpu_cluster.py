@@ -1,0 +1,85 @@
+"""End-to-end parity: cluster[] and result.fasta of the HIP path against the oracle."""
+import filecmp
+import os
+
+import numpy as np
+import pytest
+
+import gsdr
+import oracle_ffi as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(engine, tmp_path, recs, t, mode=gsdr.MODE_PLAIN, k=0):
+    fa = str(tmp_path / "in.fasta")
+    recs.write_fasta(fa)
+    ocl, operm, ost = O.cluster_file(fa, t, k=k, out_path=str(tmp_path / "orc.fasta"))
+    r = gsdr.Records.read_fasta(fa)
+    perm = r.sort()
+    assert np.array_equal(perm, operm)
+    cl, st = gsdr.cluster_records(engine, r, t, word_length=k, mode=mode)
+    r.write_result(cl, str(tmp_path / "gpu.fasta"))
+    return cl, ocl, st, ost, filecmp.cmp(str(tmp_path / "gpu.fasta"), str(tmp_path / "orc.fasta"), shallow=False)
+
+
+@pytest.mark.parametrize("config,n,t", [(1, 3000, 0.95), (1, 10000, 0.95), (2, 4000, 0.90), (2, 3000, 0.80),
+                                        (4, 4000, 0.95), (4, 4000, 0.80), (1, 2000, 0.99), (2, 2000, 0.85)])
+def test_plain_matches_oracle(engine, tmp_path, config, n, t):
+    recs = gsdr.Records.generate(config, 1000 + config, n)
+    cl, ocl, st, ost, same_file = run_both(engine, tmp_path, recs, t)
+    bad = np.nonzero(cl != ocl)[0]
+    assert len(bad) == 0, f"{len(bad)} assignments differ, first at {bad[:5]}"
+    assert same_file
+    assert st.reps == ost["reps"]
+    # stage accounting: every considered pair is rejected by exactly one stage or aligned
+    assert st.pairs == st.rej_length + st.rej_comp + st.rej_word + st.aligned
+    assert st.aligned == st.accepted + st.rej_align
+
+
+def test_iupac_crlf_edge_set(engine, tmp_path):
+    rng = np.random.default_rng(5)
+    lines = []
+    base = "".join(rng.choice(list("ACGT"), 700))
+    for i in range(300):
+        s = list(base[: int(rng.integers(200, 700))])
+        for _ in range(int(rng.integers(0, 12))):
+            s[int(rng.integers(0, len(s)))] = rng.choice(list("NRYSWKMacgtu"))
+        for _ in range(int(rng.integers(0, 10))):
+            s[int(rng.integers(0, len(s)))] = rng.choice(list("ACGT"))
+        seq = "".join(s)
+        lines.append(f">r{i}\r\n")
+        for p in range(0, len(seq), 70):
+            lines.append(seq[p:p + 70] + ("\r\n" if i % 2 else "\n"))
+        if i % 17 == 0:
+            lines.append("\n")
+    fa = tmp_path / "edge.fasta"
+    fa.write_text("".join(lines))
+    for t in (0.95, 0.85):
+        ocl, operm, _ = O.cluster_file(str(fa), t, out_path=str(tmp_path / "o.fasta"))
+        r = gsdr.Records.read_fasta(str(fa))
+        perm = r.sort()
+        assert np.array_equal(perm, operm)
+        cl, _ = gsdr.cluster_records(engine, r, t)
+        assert np.array_equal(cl, ocl)
+        r.write_result(cl, str(tmp_path / "g.fasta"))
+        assert filecmp.cmp(str(tmp_path / "g.fasta"), str(tmp_path / "o.fasta"), shallow=False)
+
+
+def test_single_record_and_ties(engine, tmp_path):
+    fa = tmp_path / "one.fasta"
+    fa.write_text(">only\nACGTACGTAC\n")
+    r = gsdr.Records.read_fasta(str(fa))
+    r.sort()
+    cl, st = gsdr.cluster_records(engine, r, 0.95)
+    assert list(cl) == [0] and st.reps == 1
+    # many equal lengths: the tie order must follow libstdc++ std::sort
+    rng = np.random.default_rng(9)
+    recs = [("".join(rng.choice(list("ACGT"), 400))) for _ in range(40)]
+    fa2 = tmp_path / "ties.fasta"
+    fa2.write_text("".join(f">t{i}\n{s}\n" for i, s in enumerate(recs * 5)))
+    ocl, operm, _ = O.cluster_file(str(fa2), 0.95)
+    r2 = gsdr.Records.read_fasta(str(fa2))
+    assert np.array_equal(r2.sort(), operm)
+    cl2, _ = gsdr.cluster_records(engine, r2, 0.95)
+    assert np.array_equal(cl2, ocl)
