@@ -119,6 +119,7 @@ void stats_from_device(const unsigned long long* d, gsdr_stats* o) {
   o->cells_full += (double)d[ST_CELLS_FULL];
   o->filter_bytes += (double)d[ST_FILTER_BYTES];
   o->spec_wasted += (double)d[ST_SPEC_WASTE];
+  o->cells_decided += (double)d[ST_CELLS_DECIDED];
 }
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {

@@ -36,6 +36,28 @@ __host__ __device__ inline double band_cells(int n, int m, int c) {
   return (double)c * (double)(n - c) + (double)n * (double)(m - c) + (double)c;
 }
 
+// Band cells of rows 1..R only (R <= m): the band work an exact decision
+// taken after R query rows needs.
+__host__ __device__ inline double band_cells_rows(int n, int m, int c, int R) {
+  if (c < 1 || c > n || c > m || R <= 0) return 0.0;
+  if (R > m) R = m;
+  const double r = R;
+  // sum of hi(j) = min(n, j + n - c)
+  const double rc = R < c ? R : c;
+  double hi = rc * (rc + 1) / 2.0 + rc * (double)(n - c) + (R > c ? (double)(R - c) * n : 0.0);
+  // sum of lo(j) = max(1, j - (m - c))
+  const int d = m - c;  // rows with lo = 1: j <= d + 1
+  double lo;
+  if (R <= d + 1) {
+    lo = r;
+  } else {
+    const double a = d + 1;                   // rows with lo == 1
+    const double u0 = 2.0, u1 = (double)(R - d);  // lo values of the other rows
+    lo = a + (u0 + u1) * (u1 - u0 + 1) / 2.0;
+  }
+  return hi - lo + r;
+}
+
 // Per-round state kept on the device (plain form).
 struct RoundState {
   int32_t rep;   // current representative, -1 when done
@@ -46,7 +68,8 @@ struct RoundState {
 // Counters accumulated on the device.
 enum StatSlot {
   ST_PAIRS = 0, ST_REJ_LEN, ST_REJ_COMP, ST_REJ_WORD, ST_ALIGNED, ST_ACCEPT, ST_REJ_ALIGN,
-  ST_CELLS_EXEC, ST_CELLS_BAND, ST_CELLS_FULL, ST_FILTER_BYTES, ST_SPEC_WASTE, ST_NUM = 16
+  ST_CELLS_EXEC, ST_CELLS_BAND, ST_CELLS_FULL, ST_FILTER_BYTES, ST_SPEC_WASTE, ST_CELLS_DECIDED,
+  ST_NUM = 16
 };
 
 // Store arrays resident in HBM (indexed by sorted sequence index).

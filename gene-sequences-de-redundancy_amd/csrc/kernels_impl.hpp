@@ -197,7 +197,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
   bool exhausted = false;
   int tx = 0, q = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
   int64_t toff = 0, qoff = 0;
-  unsigned long long cells = 0, nacc = 0, nrej = 0;
+  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0;
   const int total = *a.count;
 
   while (true) {
@@ -336,6 +336,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
       if (sl == 0) cells += (unsigned long long)S * W * 32ull * 32ull;
       if (accept || reject) {
         if (sl == 0) {
+          if (!hopeless) cdec += (unsigned long long)band_cells_rows(n, mq, c, jb * 32 + 32);
           if (accept && a.cluster) a.cluster[q] = tx;
           if (a.result) a.result[item] = accept ? 1 : 0;
           if (accept) ++nacc; else ++nrej;
@@ -352,9 +353,11 @@ __device__ inline void align_loop(const AlignArgs& a) {
     cells += __shfl_xor(cells, off);
     nacc += __shfl_xor(nacc, off);
     nrej += __shfl_xor(nrej, off);
+    cdec += __shfl_xor(cdec, off);
   }
   if (lane == 0 && a.stats) {
     if (cells) atomicAdd(&a.stats[ST_CELLS_EXEC], cells);
+    if (cdec) atomicAdd(&a.stats[ST_CELLS_DECIDED], cdec);
     if (nacc) atomicAdd(&a.stats[ST_ACCEPT], nacc);
     if (nrej) atomicAdd(&a.stats[ST_REJ_ALIGN], nrej);
   }

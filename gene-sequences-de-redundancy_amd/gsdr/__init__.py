@@ -57,7 +57,8 @@ class Stats(C.Structure):
         ("align_launches", C.c_int64),
         ("filter_bytes", C.c_double),
         ("spec_wasted", C.c_double),
-        ("reserved", C.c_double * 4),
+        ("cells_decided", C.c_double),
+        ("reserved", C.c_double * 3),
     ]
 
     def as_dict(self):

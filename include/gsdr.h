@@ -77,7 +77,8 @@ typedef struct gsdr_stats {
   int64_t align_launches;
   double filter_bytes;  /* algorithmic bytes read+written by the filter kernels */
   double spec_wasted;   /* batched: pair evaluations discarded by speculation */
-  double reserved[4];
+  double cells_decided; /* band cells of the rows each exact decision needed (32-row steps) */
+  double reserved[3];
 } gsdr_stats;
 
 /* ---- records (host side): replaces readFile/saveFile (func.cpp:70-112,

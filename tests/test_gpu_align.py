@@ -57,7 +57,7 @@ def make_pairs(seed, count, max_len):
         if rng.random() < 0.2 and len(q) > 4:
             qb = bytearray(q)
             for _ in range(int(rng.integers(1, 4))):
-                qb[int(rng.integers(0, len(qb)))] = ord(rng.choice(list(b"NRY")))
+                qb[int(rng.integers(0, len(qb)))] = int(rng.choice(list(b"NRY")))
             q = bytes(qb)
         seqs.append(text)
         seqs.append(q)
