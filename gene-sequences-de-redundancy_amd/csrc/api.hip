@@ -171,16 +171,18 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return fail(GSDR_E_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-  HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(RoundState) * 4, hipHostMallocDefault), "hipHostMalloc");
+  HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * 4, hipHostMallocDefault), "hipHostMalloc");
+  c->h_state_cap = 4;
   *out = c.release();
   return GSDR_OK;
 }
 
 void gsdr_close(gsdr_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamDestroy(c->stream);
-  if (c->h_state) hipHostFree(c->h_state);
+  (void)hipSetDevice(c->device);
+  workspace_release(c);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->h_state) (void)hipHostFree(c->h_state);
   delete c;
 }
 

@@ -1,19 +1,240 @@
-// Batched form (persistent rounds) and multi-job entry.
+// Batched form: speculative multi-representative steps for one or many jobs,
+// captured as a HIP graph so the host synchronises once per graph replay
+// instead of once per representative.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
 
 #include "driver.hpp"
 #include "kernels_impl.hpp"
 
 namespace gsdr {
 
+#define HIP_TRY(expr, what)                          \
+  do {                                               \
+    hipError_t _e = (expr);                          \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+namespace {
+
+struct StepLaunch {
+  FilterArgs fa;
+  ResolveArgs ra;
+  AlignArgs aa[kNumShapes];
+  int smax = 0;
+  int max_k = 4;
+  int filter_grid = 1;
+  int align_grid = 1;
+};
+
+// One step for every job: filter, alignment per tile shape, resolve.
+hipError_t launch_step(const StepLaunch& L, hipStream_t q, hipEvent_t* ev) {
+  hipError_t e;
+  if (ev && (e = hipEventRecord(ev[0], q)) != hipSuccess) return e;
+  if ((e = launch_filter(L.fa, L.max_k, L.filter_grid, q)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
+  for (int sh = 0; sh <= L.smax; ++sh)
+    if ((e = launch_align(sh, L.aa[sh], L.align_grid, q)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
+  return launch_resolve(L.ra, q);
+}
+
+}  // namespace
+
 int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
                     int32_t* const* out, gsdr_stats* stats) {
-  (void)c; (void)njobs; (void)stores; (void)params; (void)out; (void)stats;
-  return fail(GSDR_E_INVALID, "batched mode is not available in this build");
+  if (njobs == 0) {
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return GSDR_OK;
+  }
+  if (njobs > kMaxJobsPerLaunch) return fail(GSDR_E_LIMIT, "more than 1024 jobs in one call");
+  HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+  int st;
+  int max_k = 4, smax = 0, timing = 0, spec = 0;
+  int64_t total_n = 0, total_bases = 0, max_n = 0;
+  std::vector<JobDev> jobs((size_t)njobs);
+  for (int j = 0; j < njobs; ++j) {
+    gsdr_store& s = *stores[j];
+    if (s.ctx != c) return fail(GSDR_E_INVALID, "store belongs to another context");
+    int k = 0;
+    if ((st = check_params(s, params[j], &k))) return st;
+    if ((st = store_set_threshold(s, params[j].threshold))) return st;
+    jobs[j] = s.job();
+    max_k = std::max(max_k, k);
+    smax = std::max(smax, max_shape_for(s.max_m));
+    total_n += s.n;
+    total_bases += s.bases;
+    max_n = std::max<int64_t>(max_n, s.n);
+    timing |= params[j].timing;
+    spec = std::max(spec, params[j].spec_reps);
+  }
+  for (int j = 0; j + 1 < njobs; ++j)
+    for (int i = j + 1; i < njobs; ++i)
+      if (stores[j] == stores[i]) return fail(GSDR_E_INVALID, "the same store appears twice");
+  const int k_max = spec > 0 ? std::min(spec, kMaxSpec) : kMaxSpec;
+  // alignment list capacity per shape: enough for several speculative
+  // representatives over every job's candidates
+  const int64_t item_cap = std::min<int64_t>(std::max<int64_t>(8 * total_n, max_n), 32ll << 20);
+  if ((st = workspace_reserve(c, item_cap, njobs))) return st;
+  Workspace& w = c->ws;
+  hipStream_t q = c->stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  int32_t left = 0;
+  for (auto& jd : jobs) left += jd.n > 0;
+  HIP_TRY(hipMemcpyAsync(w.jobs, jobs.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice, q), "job table");
+  HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
+  HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
+  HIP_TRY(launch_init_jobs(w.jobs, jobs.data(), njobs, k_max, w.jobs_left, q), "init");
+
+  StepLaunch L;
+  L.max_k = max_k;
+  L.smax = smax;
+  L.filter_grid = c->cu_count * 8;
+  L.align_grid = c->cu_count * 4;
+  L.fa.jobs = w.jobs;
+  L.fa.njobs = njobs;
+  L.fa.min_shape = 0;
+  L.fa.items = w.items;
+  L.fa.item_cap = w.item_cap;
+  L.fa.bin_count = w.counters;
+  L.fa.stats = w.stats;
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    AlignArgs& aa = L.aa[sh];
+    aa.jobs = w.jobs;
+    aa.items = w.items + (int64_t)sh * w.item_cap;
+    aa.count = w.counters + sh;
+    aa.work = w.counters + kNumShapes + sh;
+    aa.result = nullptr;
+    aa.stats = w.stats;
+  }
+  L.ra.jobs = w.jobs;
+  L.ra.njobs = njobs;
+  L.ra.counters = w.counters;
+  L.ra.ncounters = 2 * kNumShapes;
+  L.ra.jobs_left = w.jobs_left;
+  L.ra.stats = w.stats;
+  L.ra.pair_budget = w.item_cap;
+
+  double ms_filter = 0, ms_align = 0;
+  int64_t launches = 2, replays = 0, steps_launched = 0;
+  // a job needs at most n steps; every step fixes at least one representative
+  const int64_t step_limit = max_n + 2;
+  if (timing) {
+    // timed form: the same step launched directly with events around the
+    // filter and the alignment kernels
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    while (true) {
+      HIP_TRY(launch_step(L, q, ev), "step launch");
+      launches += 2 + L.smax + 1;
+      ++steps_launched;
+      HIP_TRY(hipMemcpyAsync(c->h_state, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
+      HIP_TRY(hipStreamSynchronize(q), "step sync");
+      float a = 0, b = 0;
+      HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
+      HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
+      ms_filter += a;
+      ms_align += b;
+      if (*reinterpret_cast<int32_t*>(c->h_state) == 0) break;
+      if (steps_launched > step_limit) return fail(GSDR_E_DEVICE, "steps did not converge");
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  } else {
+    // graph form: B steps per replay, one host synchronisation per replay
+    const int B = 16;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (int b = 0; b < B; ++b) {
+      hipError_t e = launch_step(L, q, nullptr);
+      if (e != hipSuccess) {
+        hipGraph_t g2;
+        (void)hipStreamEndCapture(q, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        return hip_fail(e, "capture step");
+      }
+    }
+    HIP_TRY(hipStreamEndCapture(q, &graph), "end capture");
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    int32_t* h_left = reinterpret_cast<int32_t*>(c->h_state);
+    int rc = GSDR_OK;
+    while (true) {
+      hipError_t e = hipGraphLaunch(exec, q);
+      if (e == hipSuccess) e = hipMemcpyAsync(h_left, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q);
+      if (e == hipSuccess) e = hipStreamSynchronize(q);
+      if (e != hipSuccess) { rc = hip_fail(e, "graph replay"); break; }
+      ++replays;
+      steps_launched += B;
+      launches += (int64_t)B * (2 + L.smax + 1);
+      if (*h_left == 0) break;
+      if (steps_launched > step_limit + B) { rc = fail(GSDR_E_DEVICE, "steps did not converge"); break; }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+  }
+  for (int j = 0; j < njobs; ++j)
+    HIP_TRY(hipMemcpyAsync(out[j], jobs[j].cluster, (size_t)jobs[j].n * sizeof(int32_t), hipMemcpyDeviceToHost, q),
+            "cluster copy");
+  unsigned long long hst[ST_NUM];
+  HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
+  std::vector<JobState> hs((size_t)njobs);
+  for (int j = 0; j < njobs; ++j)
+    HIP_TRY(hipMemcpyAsync(&hs[j], jobs[j].state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
+  HIP_TRY(hipStreamSynchronize(q), "final sync");
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats_from_device(hst, stats);
+    stats->n_seqs = total_n;
+    stats->n_bases = total_bases;
+    int64_t reps = 0, steps = 0;
+    for (int j = 0; j < njobs; ++j) {
+      for (int i = 0; i < jobs[j].n; ++i) reps += out[j][i] == i;
+      steps = std::max<int64_t>(steps, hs[j].steps);
+    }
+    stats->reps = reps;
+    stats->rounds = steps;
+    stats->launches = launches;
+    stats->ms_total = ms;
+    stats->ms_filter = ms_filter;
+    stats->ms_align = ms_align;
+    stats->filter_launches = steps_launched;
+    stats->align_launches = steps_launched * (L.smax + 1);
+  }
+  return GSDR_OK;
 }
 
 int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
                  int32_t* const* out, gsdr_stats* stats) {
+  for (int j = 0; j < njobs; ++j)
+    if (params[j].mode != GSDR_MODE_BATCHED && params[j].mode != GSDR_MODE_PLAIN)
+      return fail(GSDR_E_INVALID, "unknown mode");
+  if (njobs > 0 && params[0].mode == GSDR_MODE_PLAIN) {
+    // plain form: the jobs one after another, one representative per round
+    gsdr_stats total;
+    std::memset(&total, 0, sizeof(total));
+    for (int j = 0; j < njobs; ++j) {
+      gsdr_stats s1;
+      int st = cluster_plain(c, *stores[j], params[j], out[j], &s1);
+      if (st) return st;
+      total.n_seqs += s1.n_seqs; total.n_bases += s1.n_bases; total.reps += s1.reps; total.rounds += s1.rounds;
+      total.launches += s1.launches; total.pairs += s1.pairs; total.rej_length += s1.rej_length;
+      total.rej_comp += s1.rej_comp; total.rej_word += s1.rej_word; total.aligned += s1.aligned;
+      total.rej_align += s1.rej_align; total.accepted += s1.accepted; total.cells_band += s1.cells_band;
+      total.cells_exec += s1.cells_exec; total.cells_full += s1.cells_full; total.ms_total += s1.ms_total;
+      total.ms_filter += s1.ms_filter; total.ms_align += s1.ms_align; total.filter_launches += s1.filter_launches;
+      total.align_launches += s1.align_launches; total.filter_bytes += s1.filter_bytes;
+      total.cells_decided += s1.cells_decided;
+    }
+    if (stats) *stats = total;
+    return GSDR_OK;
+  }
   return cluster_batched(c, njobs, stores, params, out, stats);
 }
 

@@ -1,4 +1,4 @@
-// Store construction and the plain per-round driver.
+// Store construction, launch workspace and the plain per-round driver.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,12 +18,12 @@ namespace gsdr {
   } while (0)
 
 template <typename T>
-static int dev_alloc(gsdr_store& s, T** p, size_t count) {
+static int dev_alloc(std::vector<void*>& allocs, int64_t& footprint, T** p, size_t count) {
   size_t bytes = std::max<size_t>(count * sizeof(T), 16);
   hipError_t e = hipMalloc((void**)p, bytes);
   if (e != hipSuccess) return fail(GSDR_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  s.allocs.push_back((void*)*p);
-  s.footprint += (int64_t)bytes;
+  allocs.push_back((void*)*p);
+  footprint += (int64_t)bytes;
   return GSDR_OK;
 }
 
@@ -38,23 +38,16 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
   s.max_m = p.max_m;
   s.raw_len = p.raw_len;
   s.m = p.m;
+  auto& A = s.allocs;
+  auto& F = s.footprint;
   int st;
-  if ((st = dev_alloc(s, &s.pk, (size_t)p.pk_off[n])) ||
-      (st = dev_alloc(s, &s.pkoff, (size_t)n + 1)) ||
-      (st = dev_alloc(s, &s.m_d, (size_t)n)) ||
-      (st = dev_alloc(s, &s.comp, (size_t)n)) ||
-      (st = dev_alloc(s, &s.kp, (size_t)p.kp_off[n])) ||
-      (st = dev_alloc(s, &s.kpoff, (size_t)n + 1)) ||
-      (st = dev_alloc(s, &s.bc, (size_t)n)) ||
-      (st = dev_alloc(s, &s.wc, (size_t)n)) ||
-      (st = dev_alloc(s, &s.cluster, (size_t)n)) ||
-      (st = dev_alloc(s, &s.cand[0], (size_t)n)) ||
-      (st = dev_alloc(s, &s.cand[1], (size_t)n)) ||
-      (st = dev_alloc(s, &s.items, (size_t)n * kNumShapes)) ||
-      (st = dev_alloc(s, &s.counters, (size_t)2 * kNumShapes)) ||
-      (st = dev_alloc(s, &s.bcount, (size_t)(n + 1023) / 1024 + 1)) ||
-      (st = dev_alloc(s, &s.st, (size_t)1)) ||
-      (st = dev_alloc(s, &s.stats, (size_t)ST_NUM)))
+  if ((st = dev_alloc(A, F, &s.pk, (size_t)p.pk_off[n])) || (st = dev_alloc(A, F, &s.pkoff, (size_t)n + 1)) ||
+      (st = dev_alloc(A, F, &s.m_d, (size_t)n)) || (st = dev_alloc(A, F, &s.comp, (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.kp, (size_t)p.kp_off[n])) || (st = dev_alloc(A, F, &s.kpoff, (size_t)n + 1)) ||
+      (st = dev_alloc(A, F, &s.bc, (size_t)n)) || (st = dev_alloc(A, F, &s.wc, (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.state, (size_t)1)))
     return st;
   hipStream_t q = c->stream;
   HIP_TRY(hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload pk");
@@ -69,10 +62,45 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
 }
 
 void store_release(gsdr_store& s) {
-  if (s.ctx) hipSetDevice(s.ctx->device);
-  for (void* p : s.allocs) hipFree(p);
+  if (s.ctx) (void)hipSetDevice(s.ctx->device);
+  for (void* p : s.allocs) (void)hipFree(p);
   s.allocs.clear();
   s.footprint = 0;
+}
+
+int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs) {
+  Workspace& w = c->ws;
+  if (njobs > kMaxJobsPerLaunch) return fail(GSDR_E_LIMIT, "more than 1024 jobs in one launch");
+  if (w.items && w.item_cap >= item_cap && w.jobs_cap >= njobs) return GSDR_OK;
+  workspace_release(c);
+  std::vector<void*> A;
+  int64_t F = 0;
+  int st;
+  const int64_t cap = std::max<int64_t>(item_cap, 1024);
+  if ((st = dev_alloc(A, F, &w.items, (size_t)cap * kNumShapes)) ||
+      (st = dev_alloc(A, F, &w.counters, (size_t)2 * kNumShapes)) ||
+      (st = dev_alloc(A, F, &w.stats, (size_t)ST_NUM)) ||
+      (st = dev_alloc(A, F, &w.jobs, (size_t)std::max(njobs, 1))) || (st = dev_alloc(A, F, &w.jobs_left, (size_t)1))) {
+    for (void* p : A) (void)hipFree(p);
+    w = Workspace();
+    return st;
+  }
+  w.item_cap = cap;
+  w.jobs_cap = std::max(njobs, 1);
+  w.bytes = F;
+  if (c->h_state_cap < w.jobs_cap) {
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * w.jobs_cap, hipHostMallocDefault), "hipHostMalloc");
+    c->h_state_cap = w.jobs_cap;
+  }
+  return GSDR_OK;
+}
+
+void workspace_release(gsdr_ctx* c) {
+  Workspace& w = c->ws;
+  for (void* p : {(void*)w.items, (void*)w.counters, (void*)w.stats, (void*)w.jobs, (void*)w.jobs_left})
+    if (p) (void)hipFree(p);
+  w = Workspace();
 }
 
 // Cutoffs for threshold t (func.cpp:339-374), computed on the host with the
@@ -132,107 +160,106 @@ int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
 }
 
 // Largest tile shape a representative of gap-free length n can need.
-static int max_shape_for(int n) {
+int max_shape_for(int n) {
   const int ntw = (n >> 5) + 1;
   for (int i = 0; i < kNumShapes; ++i)
     if (kShapes[i].s * kShapes[i].w >= ntw) return i;
   return kNumShapes - 1;
 }
 
-// Plain form: one representative per round. Per round the host launches the
-// filter, the alignment kernels of the tile shapes this representative can
-// need, and the compaction, then reads back the next representative (one
-// synchronisation per round, like the reference's per-round waits at
-// func.cpp:695-781 but without the host compaction loops).
+// Plain form: one representative per round (k_spec = 1). Per round the host
+// launches the filter, the alignment kernels of the tile shapes this
+// representative can need, and the resolve/compaction step, then reads back
+// the job state (one synchronisation per round, like the reference's
+// per-round waits at func.cpp:695-781 but without its host compaction).
 int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
   int k = 0;
   int st = check_params(s, p, &k);
   if (st) return st;
   HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
-  st = store_set_threshold(s, p.threshold);
-  if (st) return st;
+  if ((st = store_set_threshold(s, p.threshold))) return st;
+  if ((st = workspace_reserve(c, std::max<int64_t>(s.n, 1), 1))) return st;
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t q = c->stream;
+  Workspace& w = c->ws;
   const int n = s.n;
-  HIP_TRY(hipMemsetAsync(s.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
-  HIP_TRY(hipMemsetAsync(s.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
-  HIP_TRY(launch_init(s.cluster, s.cand[0], n, s.st, q), "init");
-  int64_t launches = 3;
+  const JobDev jd = s.job();
+  HIP_TRY(hipMemcpyAsync(w.jobs, &jd, sizeof(JobDev), hipMemcpyHostToDevice, q), "job table");
+  HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
+  HIP_TRY(launch_init_jobs(w.jobs, &jd, 1, 1, nullptr, q), "init");
+  int64_t launches = 1;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   if (p.timing)
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
   double ms_filter = 0, ms_align = 0;
-  int64_t filter_launches = 0, align_launches = 0;
-  int rep = 0, nc = n - 1, cur = 0;
-  int64_t rounds = 0;
+  int64_t filter_launches = 0, align_launches = 0, rounds = 0;
+  int nc = n, first = 0;
+  bool done = n == 0;
   const int cap_blocks = c->cu_count * 4;
-  const StoreView view = s.view();
-  while (rep >= 0) {
+  while (!done) {
     ++rounds;
-    if (nc > 0) {
-      if (p.timing) HIP_TRY(hipEventRecord(ev[0], q), "event");
-      FilterArgs fa;
-      fa.st = view;
-      fa.cand = s.cand[cur];
-      fa.ncp = &s.st->nc;
-      fa.repp = &s.st->rep;
-      fa.items = s.items;
-      fa.item_cap = n;
-      fa.min_shape = 0;
-      fa.bin_count = s.counters;
-      fa.stats = s.stats;
-      const int fgrid = std::min((nc + 255) / 256, cap_blocks);
-      HIP_TRY(launch_filter(k, fa, fgrid, q), "filter launch");
+    const int units = (nc + kFilterChunk - 1) / kFilterChunk;
+    if (p.timing) HIP_TRY(hipEventRecord(ev[0], q), "event");
+    FilterArgs fa;
+    fa.jobs = w.jobs;
+    fa.njobs = 1;
+    fa.min_shape = 0;
+    fa.items = w.items;
+    fa.item_cap = w.item_cap;
+    fa.bin_count = w.counters;
+    fa.stats = w.stats;
+    HIP_TRY(launch_filter(fa, k, std::max(1, std::min(units, cap_blocks * 2)), q), "filter launch");
+    ++launches;
+    ++filter_launches;
+    if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
+    const int smax = max_shape_for(s.m[first]);
+    for (int sh = 0; sh <= smax; ++sh) {
+      AlignArgs aa;
+      aa.jobs = w.jobs;
+      aa.items = w.items + (int64_t)sh * w.item_cap;
+      aa.count = w.counters + sh;
+      aa.work = w.counters + kNumShapes + sh;
+      aa.result = nullptr;
+      aa.stats = w.stats;
+      const int segs = 64 / kShapes[sh].s;
+      const int waves = (nc + segs - 1) / segs;
+      HIP_TRY(launch_align(sh, aa, std::max(1, std::min((waves + 3) / 4, cap_blocks)), q), "align launch");
       ++launches;
-      ++filter_launches;
-      if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
-      const int smax = max_shape_for(s.m[rep]);
-      for (int sh = 0; sh <= smax; ++sh) {
-        AlignArgs aa;
-        aa.st = view;
-        aa.items = s.items + (int64_t)sh * n;
-        aa.count = s.counters + sh;
-        aa.work = s.counters + kNumShapes + sh;
-        aa.cluster = s.cluster;
-        aa.result = nullptr;
-        aa.stats = s.stats;
-        const int segs = 64 / kShapes[sh].s;
-        const int waves = (nc + segs - 1) / segs;
-        const int grid = std::max(1, std::min((waves + 3) / 4, cap_blocks));
-        HIP_TRY(launch_align(sh, aa, grid, q), "align launch");
-        ++launches;
-        ++align_launches;
-      }
-      if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
+      ++align_launches;
     }
-    HIP_TRY(launch_compact(s.cand[cur], s.st, s.cluster, s.bcount, s.cand[1 - cur], nc, s.counters,
-                           2 * kNumShapes, q),
-            "compact launch");
-    launches += 2;
-    HIP_TRY(hipMemcpyAsync(c->h_state, s.st, sizeof(RoundState), hipMemcpyDeviceToHost, q), "state copy");
+    if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
+    ResolveArgs ra;
+    ra.jobs = w.jobs;
+    ra.njobs = 1;
+    ra.counters = w.counters;
+    ra.ncounters = 2 * kNumShapes;
+    ra.jobs_left = nullptr;
+    ra.stats = w.stats;
+    ra.pair_budget = w.item_cap;
+    HIP_TRY(launch_resolve(ra, q), "resolve launch");
+    ++launches;
+    HIP_TRY(hipMemcpyAsync(c->h_state, s.state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
     HIP_TRY(hipStreamSynchronize(q), "round sync");
-    if (p.timing && nc > 0) {
+    if (p.timing) {
       float a = 0, b = 0;
       HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
       HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
       ms_filter += a;
       ms_align += b;
     }
-    const int next = c->h_state->rep;
-    const int next_nc = c->h_state->nc;
-    // each round fixes one representative and shrinks the candidate list
-    if (next >= 0 && (next <= rep || next_nc >= nc || rounds > (int64_t)n))
-      return fail(GSDR_E_DEVICE, "round state did not advance");
-    rep = next;
-    nc = next_nc;
-    cur ^= 1;
+    const JobState& h = c->h_state[0];
+    if (!h.done && (h.nc >= nc || rounds > (int64_t)n)) return fail(GSDR_E_DEVICE, "round state did not advance");
+    nc = h.nc;
+    first = h.first;
+    done = h.done != 0;
   }
   HIP_TRY(hipMemcpyAsync(out, s.cluster, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, q), "cluster copy");
   unsigned long long hst[ST_NUM];
-  HIP_TRY(hipMemcpyAsync(hst, s.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
+  HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");
   if (p.timing)
-    for (auto& e : ev) hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e);
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -272,10 +299,8 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     return GSDR_OK;
   }
   const int min_shape = shape < 0 ? 0 : std::min(shape, kNumShapes - 1);
-  // a private store: sequences, plus per-pair cutoffs indexed through a
-  // per-pair query copy (query slot = nseq + pair) so each pair has its bc.
-  gsdr_store s;
-  s.ctx = c;
+  // a private store: sequences, plus one query slot per pair (nseq + pair)
+  // that points at the query's packed words and carries the pair's cutoff
   const int64_t nslots = (int64_t)nseq + npairs;
   std::vector<int64_t> pkoff((size_t)nslots + 1);
   std::vector<int32_t> m((size_t)nslots), bcv((size_t)nslots, 0);
@@ -286,27 +311,44 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     m[nseq + i] = p.m[qq];
     bcv[nseq + i] = bc[i];
   }
-  std::vector<std::vector<int2>> bins(kNumShapes);
+  std::vector<std::vector<AlignItem>> bins(kNumShapes);
   std::vector<std::vector<int>> bin_pair(kNumShapes);
   for (int i = 0; i < npairs; ++i) {
     const int t = pairs[2 * i];
     const int sh = pick_shape(p.m[t], m[nseq + i], bcv[nseq + i] + 1, min_shape);
-    bins[sh].push_back(make_int2(t, nseq + i));
+    AlignItem it;
+    it.text = t;
+    it.query = nseq + i;
+    it.job = 0;
+    it.slot = 0;
+    bins[sh].push_back(it);
     bin_pair[sh].push_back(i);
   }
+  std::vector<void*> A;
+  int64_t F = 0;
+  uint2* d_pk = nullptr;
+  int64_t* d_pkoff = nullptr;
+  int32_t *d_m = nullptr, *d_bc = nullptr, *res = nullptr, *ctr = nullptr;
+  AlignItem* items = nullptr;
+  JobDev* d_job = nullptr;
+  unsigned long long* d_stats = nullptr;
+  auto release = [&]() { for (void* x : A) (void)hipFree(x); };
   int st;
-  int2* items = nullptr;
-  int32_t* res = nullptr;
-  int32_t* ctr = nullptr;
-  if ((st = dev_alloc(s, &s.pk, p.pk.size() / 2)) || (st = dev_alloc(s, &s.pkoff, (size_t)nslots + 1)) ||
-      (st = dev_alloc(s, &s.m_d, (size_t)nslots)) || (st = dev_alloc(s, &s.bc, (size_t)nslots)) ||
-      (st = dev_alloc(s, &items, (size_t)npairs)) || (st = dev_alloc(s, &res, (size_t)npairs)) ||
-      (st = dev_alloc(s, &ctr, (size_t)2 * kNumShapes)) || (st = dev_alloc(s, &s.stats, (size_t)ST_NUM))) {
-    store_release(s);
+  if ((st = dev_alloc(A, F, &d_pk, p.pk.size() / 2)) || (st = dev_alloc(A, F, &d_pkoff, (size_t)nslots + 1)) ||
+      (st = dev_alloc(A, F, &d_m, (size_t)nslots)) || (st = dev_alloc(A, F, &d_bc, (size_t)nslots)) ||
+      (st = dev_alloc(A, F, &items, (size_t)npairs)) || (st = dev_alloc(A, F, &res, (size_t)npairs)) ||
+      (st = dev_alloc(A, F, &ctr, (size_t)2 * kNumShapes)) || (st = dev_alloc(A, F, &d_job, (size_t)1)) ||
+      (st = dev_alloc(A, F, &d_stats, (size_t)ST_NUM))) {
+    release();
     return st;
   }
-  hipStream_t q = c->stream;
-  std::vector<int2> all;
+  JobDev jd;
+  std::memset(&jd, 0, sizeof(jd));
+  jd.st.pk = d_pk;
+  jd.st.pkoff = d_pkoff;
+  jd.st.m = d_m;
+  jd.st.bc = d_bc;
+  std::vector<AlignItem> all;
   std::vector<int32_t> counts(2 * kNumShapes, 0);
   std::vector<int> first(kNumShapes, 0);
   for (int sh = 0; sh < kNumShapes; ++sh) {
@@ -314,42 +356,41 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     counts[sh] = (int)bins[sh].size();
     all.insert(all.end(), bins[sh].begin(), bins[sh].end());
   }
-  auto cleanup_fail = [&](hipError_t e, const char* what) { store_release(s); return hip_fail(e, what); };
+  hipStream_t q = c->stream;
   hipError_t e;
-  if ((e = hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * 4, hipMemcpyHostToDevice, q)) ||
-      (e = hipMemcpyAsync(s.pkoff, pkoff.data(), pkoff.size() * 8, hipMemcpyHostToDevice, q)) ||
-      (e = hipMemcpyAsync(s.m_d, m.data(), m.size() * 4, hipMemcpyHostToDevice, q)) ||
-      (e = hipMemcpyAsync(s.bc, bcv.data(), bcv.size() * 4, hipMemcpyHostToDevice, q)) ||
-      (e = hipMemcpyAsync(items, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice, q)) ||
+  auto bail = [&](hipError_t err, const char* what) { release(); return hip_fail(err, what); };
+  if ((e = hipMemcpyAsync(d_pk, p.pk.data(), p.pk.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(d_pkoff, pkoff.data(), pkoff.size() * 8, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(d_m, m.data(), m.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(d_bc, bcv.data(), bcv.size() * 4, hipMemcpyHostToDevice, q)) ||
+      (e = hipMemcpyAsync(items, all.data(), all.size() * sizeof(AlignItem), hipMemcpyHostToDevice, q)) ||
       (e = hipMemcpyAsync(ctr, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, q)) ||
-      (e = hipMemsetAsync(s.stats, 0, ST_NUM * 8, q)))
-    return cleanup_fail(e, "align_pairs upload");
-  const StoreView view = s.view();
+      (e = hipMemcpyAsync(d_job, &jd, sizeof(jd), hipMemcpyHostToDevice, q)) ||
+      (e = hipMemsetAsync(d_stats, 0, ST_NUM * 8, q)))
+    return bail(e, "align_pairs upload");
   for (int sh = 0; sh < kNumShapes; ++sh) {
     if (counts[sh] == 0) continue;
     AlignArgs aa;
-    aa.st = view;
+    aa.jobs = d_job;
     aa.items = items + first[sh];
     aa.count = ctr + sh;
     aa.work = ctr + kNumShapes + sh;
-    aa.cluster = nullptr;
     aa.result = res + first[sh];
-    aa.stats = s.stats;
+    aa.stats = d_stats;
     const int segs = 64 / kShapes[sh].s;
     const int waves = (counts[sh] + segs - 1) / segs;
     const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * 4));
-    if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return cleanup_fail(e, "align launch");
+    if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return bail(e, "align launch");
   }
   std::vector<int32_t> got(all.size());
   unsigned long long hst[ST_NUM];
   if ((e = hipMemcpyAsync(got.data(), res, got.size() * 4, hipMemcpyDeviceToHost, q)) ||
-      (e = hipMemcpyAsync(hst, s.stats, sizeof(hst), hipMemcpyDeviceToHost, q)) ||
-      (e = hipStreamSynchronize(q)))
-    return cleanup_fail(e, "align_pairs download");
+      (e = hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, q)) || (e = hipStreamSynchronize(q)))
+    return bail(e, "align_pairs download");
   for (int sh = 0; sh < kNumShapes; ++sh)
     for (size_t j = 0; j < bin_pair[sh].size(); ++j) accept_out[bin_pair[sh][j]] = got[first[sh] + j];
   if (cells_exec_out) *cells_exec_out = (double)hst[ST_CELLS_EXEC];
-  store_release(s);
+  release();
   return GSDR_OK;
 }
 

@@ -15,13 +15,29 @@ struct gsdr_records {
   gsdr::Records r;
 };
 
+// Launch workspace shared by every clustering call of a context: per-shape
+// alignment lists, counters, statistics and the device job table.
+struct Workspace {
+  gsdr::AlignItem* items = nullptr;
+  int64_t item_cap = 0;            // items per shape
+  int32_t* counters = nullptr;     // [kNumShapes] bin counts, [kNumShapes] work counters
+  unsigned long long* stats = nullptr;
+  gsdr::JobDev* jobs = nullptr;    // device job table
+  int32_t jobs_cap = 0;
+  int32_t* jobs_left = nullptr;
+  int64_t bytes = 0;
+};
+
 struct gsdr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int64_t hbm_bytes = 0;
   char name[256] = {0};
-  gsdr::RoundState* h_state = nullptr;  // pinned host copy of the round state
+  gsdr::JobState* h_state = nullptr;  // pinned host copies of job states
+  int32_t h_state_cap = 0;
+  int32_t* h_left = nullptr;          // pinned copy of jobs_left
+  Workspace ws;
 };
 
 struct gsdr_store {
@@ -41,14 +57,11 @@ struct gsdr_store {
   int32_t* bc = nullptr;
   int32_t* wc = nullptr;
   float t_loaded = -1.0f;
-  // per-call workspace (allocated once with the store)
+  // per-job round state (allocated with the store)
   int32_t* cluster = nullptr;
   int32_t* cand[2] = {nullptr, nullptr};
-  int2* items = nullptr;            // kNumShapes lists of n pairs
-  int32_t* counters = nullptr;      // bin counts [kNumShapes] then work counters [kNumShapes]
-  int32_t* bcount = nullptr;        // compaction block counts
-  gsdr::RoundState* st = nullptr;
-  unsigned long long* stats = nullptr;
+  unsigned long long* acc = nullptr;
+  gsdr::JobState* state = nullptr;
   int64_t footprint = 0;
   std::vector<void*> allocs;
 
@@ -56,6 +69,18 @@ struct gsdr_store {
     gsdr::StoreView v;
     v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
     return v;
+  }
+  gsdr::JobDev job() const {
+    gsdr::JobDev j;
+    j.st = view();
+    j.cluster = cluster;
+    j.cand[0] = cand[0];
+    j.cand[1] = cand[1];
+    j.acc = acc;
+    j.state = state;
+    j.n = n;
+    j.k = k;
+    return j;
   }
 };
 
@@ -68,6 +93,9 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s);
 void store_release(gsdr_store& s);
 int store_set_threshold(gsdr_store& s, float t);
 int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out);
+int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs);
+void workspace_release(gsdr_ctx* c);
+int max_shape_for(int n);
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
 int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);

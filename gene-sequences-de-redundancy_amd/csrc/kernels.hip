@@ -1,147 +1,175 @@
 // gfx950 kernels of the per-round pipeline: shared-word/composition filter,
-// banded bit-parallel LCS decision, and order-preserving candidate
-// compaction with next-representative selection.
+// banded bit-parallel LCS decision, and the resolve step that fixes the
+// representatives of a step, assigns clusters and compacts the candidates.
 //
 // Reference stages replaced: kernel_makeTable/kernel_cleanTable,
 // kernel_preFilter, kernel_filter, kernel_dynamic (func.cpp:499-676) and the
 // host loops updateRepresentative/updateRemain/updatJobs (func.cpp:466-498).
+#define GSDR_DEVICE_TU
 #include "kernels.hpp"
 #include "kernels_impl.hpp"
 
 namespace gsdr {
 
-// ------------------------------------------------------------------ filter
-// One representative per launch. Each workgroup stages the representative's
-// word counts c_r(w) in LDS (a u16 table of 4^k entries, the reference's
-// global table at func.cpp:449), then each wave takes 64 candidates:
-//   lane-per-candidate:  length check (added, sound), composition filter
-//                        (func.cpp:525-543) plus its sound tightening;
-//   wave-per-survivor:   sum over the candidate's distinct words of
-//                        min(c_r(w), c_q(w)) against wc_q (func.cpp:544-571).
-// Survivors are appended to the alignment list of their tile shape.
-template <int K>
-__global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
-  extern __shared__ uint32_t lds[];
-  uint16_t* table = reinterpret_cast<uint16_t*>(lds);
-  constexpr int NW = 1 << (2 * K);
-  const int nc = *a.ncp;
-  const int r = *a.repp;
-  if (r < 0 || nc <= 0) return;
+// ----------------------------------------------------------------- resolve
+// One 1024-thread workgroup per job, after the step's alignments.
+//
+// Speculative step with representatives r_0 < ... < r_{K-1} (the K smallest
+// unclustered indices): the greedy recursion of func.cpp:466-784 makes r_i a
+// representative iff no earlier true representative r_j accepts it, and puts
+// every later q into the cluster of the first true representative that
+// accepts it. acc[q] bit i holds P(r_i, q). With K = 1 this is exactly one
+// reference round: r_0 is the representative and q joins it iff P(r_0, q).
+// The unassigned candidates are then compacted in order (updateRemain,
+// func.cpp:477-487), so the next step's representatives are again the
+// smallest unclustered indices (updateRepresentative, func.cpp:466-476).
+__global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
+  __shared__ int s_part[16];
+  __shared__ unsigned long long s_tm;
+  __shared__ int s_nonrep;
+  __shared__ long long s_waste;
   const int tid = threadIdx.x;
-  for (int i = tid; i < NW / 2; i += blockDim.x) lds[i] = 0u;
-  __syncthreads();
-  const int64_t k0 = a.st.kpoff[r], k1 = a.st.kpoff[r + 1];
-  for (int64_t e = k0 + tid; e < k1; e += blockDim.x) {
-    const uint32_t x = a.st.kp[e];
-    table[x & 0xFFFFu] = (uint16_t)(x >> 16);
-  }
-  __syncthreads();
-  filter_wave_loop(a, table, r, nc);
-}
-
-// --------------------------------------------------------------- compaction
-// Order-preserving removal of absorbed candidates. The first survivor is the
-// next representative (updateRepresentative, func.cpp:466-476); the rest form
-// the next candidate list (updateRemain, func.cpp:477-487).
-__global__ __launch_bounds__(256) void compact_count(const int32_t* __restrict__ cand, int nc,
-                                                     const int32_t* __restrict__ cluster,
-                                                     int32_t* __restrict__ bcount) {
-  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-  int cnt = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int i = base + k;
-    if (i < nc && cluster[cand[i]] == -1) ++cnt;
-  }
-  cnt = block_sum_256(cnt);
-  if (threadIdx.x == 0) bcount[blockIdx.x] = cnt;
-}
-
-__global__ __launch_bounds__(256) void compact_scatter(const int32_t* __restrict__ cand, int nc,
-                                                       RoundState* __restrict__ st,
-                                                       int32_t* __restrict__ cluster,
-                                                       const int32_t* __restrict__ bcount,
-                                                       int32_t* __restrict__ out, int nblocks,
-                                                       int32_t* __restrict__ reset, int nreset) {
-  // prefix over preceding blocks
-  int pre = 0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256) pre += bcount[b];
-  pre = block_sum_256(pre);
-  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-  int32_t q[4];
-  int f[4];
-  int cnt = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int i = base + k;
-    q[k] = i < nc ? cand[i] : -1;
-    f[k] = (q[k] >= 0 && cluster[q[k]] == -1) ? 1 : 0;
-    cnt += f[k];
-  }
-  int total_block = 0;
-  const int excl = block_exclusive_scan_256(cnt, &total_block);
-  int pos = pre + excl;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (f[k]) {
-      if (pos == 0) {
-        st->rep = q[k];
-        cluster[q[k]] = q[k];
-      } else {
-        out[pos - 1] = q[k];
-      }
-      ++pos;
-    }
-  }
-  if (blockIdx.x == (unsigned)(nblocks - 1) && threadIdx.x == 0) {
-    const int total = pre + total_block;
-    st->pad[0] = total;
-    if (total == 0) st->rep = -1;
-    st->nc = total > 0 ? total - 1 : 0;
-  }
   if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < nreset; i += 256) reset[i] = 0;
+    for (int i = tid; i < a.ncounters; i += 1024) a.counters[i] = 0;
+  const JobDev& J = a.jobs[blockIdx.x];
+  JobState* S = J.state;
+  if (S->done) return;
+  const int K = S->k_spec, nc = S->nc, cur = S->cur;
+  const int32_t* cand = J.cand[cur];
+  int32_t* out = J.cand[cur ^ 1];
+  if (tid == 0) {
+    unsigned long long tm = 0;
+    int nonrep = 0;
+    long long waste = 0;
+    for (int i = 0; i < K; ++i) {
+      const int r = cand[i];
+      const unsigned long long msk = J.acc[r] & tm;
+      if (msk == 0) {
+        tm |= 1ull << i;
+        J.cluster[r] = r;
+      } else {
+        J.cluster[r] = cand[__builtin_ctzll(msk)];
+        ++nonrep;
+        waste += nc - i - 1;
+      }
+      J.acc[r] = 0;
+    }
+    s_tm = tm;
+    s_nonrep = nonrep;
+    s_waste = waste;
+  }
+  __syncthreads();
+  const unsigned long long tm = s_tm;
+  int absorbed = 0;
+  int written = 0;
+  for (int base = K; base < nc; base += 4096) {
+    int qv[4], keep[4], cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + tid * 4 + k;
+      qv[k] = -1;
+      keep[k] = 0;
+      if (p < nc) {
+        const int q = cand[p];
+        qv[k] = q;
+        const unsigned long long acc = J.acc[q];
+        unsigned long long msk = 0;
+        if (acc) {
+          J.acc[q] = 0;
+          msk = acc & tm;
+        }
+        if (msk) {
+          J.cluster[q] = cand[__builtin_ctzll(msk)];
+          ++absorbed;
+        } else {
+          keep[k] = 1;
+        }
+      }
+      cnt += keep[k];
+    }
+    int tot;
+    int at = written + block_exclusive_scan<1024>(cnt, &tot, s_part);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (keep[k]) out[at++] = qv[k];
+    written += tot;
+  }
+  absorbed = block_sum<1024>(absorbed, s_part);
+  if (tid == 0) {
+    const int true_reps = __popcll(tm);
+    const int nc2 = written;
+    S->reps += true_reps;
+    S->steps += 1;
+    S->cur = cur ^ 1;
+    S->nc = nc2;
+    int k2 = 1;
+    if (S->k_max > 1) {
+      // absorption per true representative and candidate; a step of K
+      // speculative representatives wastes about K*f/2 of its pair work
+      const double denom = (double)true_reps * (double)max(nc - K, 1);
+      const double f = (double)(absorbed + s_nonrep) / denom;
+      const int up = min(2 * K, S->k_max), down = max(1, K / 2);
+      if (f <= 0.0) {
+        k2 = up;
+      } else {
+        const double want = 0.25 / f;
+        k2 = want >= up ? up : (want <= down ? down : (int)want);
+      }
+    }
+    // every survivor of a step may land in one shape list: keep k_spec * nc
+    // of all jobs within its capacity
+    const long long cap = a.pair_budget / ((long long)a.njobs * (long long)max(nc2, 1));
+    if ((long long)k2 > cap) k2 = (int)cap;
+    if (k2 > nc2) k2 = nc2;
+    if (k2 < 1) k2 = 1;
+    S->k_spec = k2;
+    S->units = nc2 > 0 ? k2 * ((nc2 + kFilterChunk - 1) / kFilterChunk) : 0;
+    S->first = nc2 > 0 ? out[0] : -1;
+    if (nc2 == 0) {
+      S->done = 1;
+      if (a.jobs_left) atomicSub(a.jobs_left, 1);
+    }
+    if (s_waste) atomicAdd(&a.stats[ST_SPEC_WASTE], (unsigned long long)s_waste);
+  }
 }
 
-// Initial state: representative 0, candidates 1..n-1, every cluster -1.
-__global__ __launch_bounds__(256) void init_round(int32_t* __restrict__ cluster,
-                                                  int32_t* __restrict__ cand, int n,
-                                                  RoundState* __restrict__ st) {
+// Initial state of every job: all clusters -1, candidates 0..n-1, one
+// representative slot.
+__global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict__ jobs, int k_max) {
+  const JobDev& J = jobs[blockIdx.y];
+  const int n = J.n;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) {
-    cluster[i] = (i == 0) ? 0 : -1;
-    if (i > 0) cand[i - 1] = i;
+    J.cluster[i] = -1;
+    J.acc[i] = 0ull;
+    J.cand[0][i] = i;
   }
-  if (i == 0) {
-    st->rep = n > 0 ? 0 : -1;
-    st->nc = n > 0 ? n - 1 : 0;
-    st->pad[0] = n;
-    st->pad[1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    JobState* S = J.state;
+    S->cur = 0;
+    S->nc = n;
+    S->k_spec = 1;
+    S->done = n > 0 ? 0 : 1;
+    S->units = n > 0 ? (n + kFilterChunk - 1) / kFilterChunk : 0;
+    S->steps = 0;
+    S->reps = 0;
+    S->k_max = k_max;
+    S->first = n > 0 ? 0 : -1;
   }
 }
 
 // -------------------------------------------------------------- launchers
-template <int K>
-static hipError_t launch_filter_k(const FilterArgs& a, int grid, hipStream_t s) {
-  const size_t lds = sizeof(uint16_t) << (2 * K);
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel<K>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+  const size_t lds = sizeof(uint16_t) << (2 * max_k);
+  static bool attr_set = false;
+  if (lds > 32768 && !attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(uint16_t) << 16));
     if (e != hipSuccess) return e;
+    attr_set = true;
   }
-  hipLaunchKernelGGL(filter_kernel<K>, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), lds, s, a);
   return hipGetLastError();
-}
-
-hipError_t launch_filter(int k, const FilterArgs& a, int grid, hipStream_t s) {
-  switch (k) {
-    case 4: return launch_filter_k<4>(a, grid, s);
-    case 5: return launch_filter_k<5>(a, grid, s);
-    case 6: return launch_filter_k<6>(a, grid, s);
-    case 7: return launch_filter_k<7>(a, grid, s);
-    case 8: return launch_filter_k<8>(a, grid, s);
-    default: return hipErrorInvalidValue;
-  }
 }
 
 template <int S, int W>
@@ -164,18 +192,17 @@ hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s) 
   }
 }
 
-hipError_t launch_compact(const int32_t* cand, RoundState* st, int32_t* cluster, int32_t* bcount,
-                          int32_t* out, int nc, int32_t* reset, int nreset, hipStream_t s) {
-  const int nblocks = nc > 0 ? (nc + 1023) / 1024 : 1;
-  hipLaunchKernelGGL(compact_count, dim3(nblocks), dim3(256), 0, s, cand, nc, cluster, bcount);
-  hipLaunchKernelGGL(compact_scatter, dim3(nblocks), dim3(256), 0, s, cand, nc, st, cluster, bcount,
-                     out, nblocks, reset, nreset);
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(resolve_kernel, dim3(a.njobs), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_init(int32_t* cluster, int32_t* cand, int n, RoundState* st, hipStream_t s) {
-  const int grid = n > 0 ? (n + 255) / 256 : 1;
-  hipLaunchKernelGGL(init_round, dim3(grid), dim3(256), 0, s, cluster, cand, n, st);
+hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
+                            int32_t* jobs_left, hipStream_t s) {
+  (void)jobs_left;
+  int max_n = 1;
+  for (int j = 0; j < njobs; ++j) max_n = max(max_n, jobs_host[j].n);
+  hipLaunchKernelGGL(init_jobs_kernel, dim3((max_n + 255) / 256, njobs), dim3(256), 0, s, jobs_dev, k_max);
   return hipGetLastError();
 }
 

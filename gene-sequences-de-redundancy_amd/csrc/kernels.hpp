@@ -58,13 +58,6 @@ __host__ __device__ inline double band_cells_rows(int n, int m, int c, int R) {
   return hi - lo + r;
 }
 
-// Per-round state kept on the device (plain form).
-struct RoundState {
-  int32_t rep;   // current representative, -1 when done
-  int32_t nc;    // candidates after rep
-  int32_t pad[2];
-};
-
 // Counters accumulated on the device.
 enum StatSlot {
   ST_PAIRS = 0, ST_REJ_LEN, ST_REJ_COMP, ST_REJ_WORD, ST_ALIGNED, ST_ACCEPT, ST_REJ_ALIGN,
@@ -82,6 +75,44 @@ struct StoreView {
   const int64_t* kpoff;    // n+1 offsets into kp
   const int32_t* bc;       // ceil(L*t) (func.cpp:368)
   const int32_t* wc;       // shared-word cutoff (func.cpp:353-366)
+};
+
+// Speculative representatives per step are bounded by the accept-mask width.
+constexpr int kMaxSpec = 64;
+// Candidates per filter work unit (one per 16-lane group of a 256-thread block).
+constexpr int kFilterChunk = 16;
+
+// Per-job round state on the device. The candidate list cand[cur][0..nc)
+// holds the unclustered sorted indices in ascending order; its first k_spec
+// entries are this step's (speculative) representatives.
+struct JobState {
+  int32_t cur;      // current candidate buffer
+  int32_t nc;       // entries in the current list
+  int32_t k_spec;   // speculative representatives this step
+  int32_t done;     // 1 when every sequence has a cluster
+  int32_t units;    // filter work units of this step: k_spec * ceil(nc / kFilterChunk)
+  int32_t steps;    // steps taken
+  int32_t reps;     // representatives fixed so far
+  int32_t k_max;    // upper bound for k_spec (1 = plain form)
+  int32_t first;    // cand[cur][0], the next representative (-1 when done)
+  int32_t pad[3];
+};
+
+// One clustering job as the kernels see it.
+struct JobDev {
+  StoreView st;
+  int32_t* cluster;          // n entries, -1 until assigned
+  int32_t* cand[2];          // candidate lists (ping-pong)
+  unsigned long long* acc;   // n accept masks: bit i = P(rep slot i, q)
+  JobState* state;
+  int32_t n;
+  int32_t k;                 // word length of the stored profiles
+};
+
+// Alignment work item: text (representative) and query sorted indices, the
+// job and the representative's slot in this step.
+struct AlignItem {
+  int32_t text, query, job, slot;
 };
 
 }  // namespace gsdr

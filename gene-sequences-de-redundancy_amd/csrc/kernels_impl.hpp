@@ -4,26 +4,35 @@
 
 namespace gsdr {
 
+constexpr int kMaxJobsPerLaunch = 1024;
+
 struct FilterArgs {
-  StoreView st;
-  const int32_t* cand;      // candidate sorted indices (ascending)
-  const int32_t* ncp;       // number of candidates (device)
-  const int32_t* repp;      // representative (device)
-  int2* items;              // per-shape pair lists: items + shape * item_cap
-  int32_t item_cap;
+  const JobDev* jobs;
+  int32_t njobs;
   int32_t min_shape;
+  AlignItem* items;         // per-shape lists: items + shape * item_cap
+  int64_t item_cap;
   int32_t* bin_count;       // kNumShapes counters
   unsigned long long* stats;
 };
 
 struct AlignArgs {
-  StoreView st;
-  const int2* items;        // (text, query) sorted indices
+  const JobDev* jobs;
+  const AlignItem* items;   // this shape's list
   const int32_t* count;     // number of items (device)
   int32_t* work;            // work counter (device, starts at 0)
-  int32_t* cluster;         // non-null: cluster[query] = text on accept
-  int32_t* result;          // non-null: result[item] = 1 accept / 0 reject
+  int32_t* result;          // non-null: result[item] = 1 accept / 0 reject (pair-level entry)
   unsigned long long* stats;
+};
+
+struct ResolveArgs {
+  const JobDev* jobs;
+  int32_t njobs;
+  int32_t* counters;        // bin counts and work counters, reset for the next step
+  int32_t ncounters;
+  int32_t* jobs_left;       // jobs not done (device)
+  unsigned long long* stats;
+  int64_t pair_budget;      // alignment list capacity per shape (bounds k_spec * nc over the jobs)
 };
 
 // ----------------------------------------------------------- block helpers
@@ -33,19 +42,22 @@ __device__ inline int wave_sum(int v) {
   return v;
 }
 
-__device__ inline int block_sum_256(int v) {
-  __shared__ int part[4];
+template <int NT>
+__device__ inline int block_sum(int v, int* part) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) part[wave] = v;
   __syncthreads();
-  return part[0] + part[1] + part[2] + part[3];
+  int s = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += part[w];
+  return s;
 }
 
-// Exclusive scan over a 256-thread block; *total gets the block sum.
-__device__ inline int block_exclusive_scan_256(int v, int* total) {
-  __shared__ int part[4];
+// Exclusive scan over an NT-thread block; *total gets the block sum.
+template <int NT>
+__device__ inline int block_exclusive_scan(int v, int* total, int* part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int incl = v;
 #pragma unroll
@@ -56,76 +68,149 @@ __device__ inline int block_exclusive_scan_256(int v, int* total) {
   __syncthreads();
   if (lane == 63) part[wave] = incl;
   __syncthreads();
-  int before = 0;
-  for (int w = 0; w < wave; ++w) before += part[w];
-  *total = part[0] + part[1] + part[2] + part[3];
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const int x = part[w];
+    if (w < wave) before += x;
+    tot += x;
+  }
+  *total = tot;
   return before + incl - v;
 }
 
-// ------------------------------------------------------------ filter body
-// Wave loop of the filter kernel; `table` holds c_r(w) for representative r.
-__device__ inline void filter_wave_loop(const FilterArgs& a, const uint16_t* table, int r, int nc) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int waves_per_block = blockDim.x >> 6;
-  const int n = a.st.m[r];
-  const uint4 cr = a.st.comp[r];
-  unsigned long long rej_len = 0, rej_comp = 0, rej_word = 0, passed = 0;
-  unsigned long long bytes = 0, cband = 0, cfull = 0;
-  for (int base = (blockIdx.x * waves_per_block + wave) * 64; base < nc;
-       base += gridDim.x * waves_per_block * 64) {
-    const int idx = base + lane;
-    const bool in = idx < nc;
-    int q = 0, mq = 0, bcq = 0;
-    bool ok = false;
-    if (in) {
-      q = a.cand[idx];
-      mq = a.st.m[q];
-      bcq = a.st.bc[q];
-      bytes += 28;  // id, m, bc, comp
-      const int c = bcq + 1;
-      if (c > min(n, mq)) {
-        ++rej_len;  // LCS <= min(n, m) < c
-      } else {
-        const uint4 cq = a.st.comp[q];
-        const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
-        // reference F1 rejects sum < bc; LCS <= sum also rejects sum == bc
-        if (sum <= bcq) ++rej_comp;
-        else ok = true;
+// Kernel bodies are compiled only in the translation unit that defines
+// GSDR_DEVICE_TU (kernels.hip); the other files see the launchers.
+#ifdef GSDR_DEVICE_TU
+// ------------------------------------------------------------------ filter
+// Work unit = (job, representative slot i, chunk of kFilterChunk candidate
+// positions). Each 16-lane group owns one candidate q at a position after
+// slot i:
+//   length check (added, sound): c = bc_q + 1 > min(n, m_q) means LCS < c;
+//   composition (func.cpp:525-543): reject sum < bc_q, plus the sound
+//     tightening sum == bc_q (LCS <= sum < bc_q + 1);
+//   shared words (func.cpp:544-571): sum over q's distinct k-mers of
+//     min(c_r(w), c_q(w)) < wc_q rejects, with c_r staged in LDS.
+// Survivors are appended to the alignment list of their tile shape.
+__global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
+  extern __shared__ uint32_t lds[];
+  uint16_t* table = reinterpret_cast<uint16_t*>(lds);
+  __shared__ int s_pre[kMaxJobsPerLaunch + 1];
+  __shared__ int s_part[4];
+  const int tid = threadIdx.x;
+  const int nj = a.njobs;
+  // prefix of work units over jobs
+  int total = 0;
+  {
+    int carry = 0;
+    for (int base = 0; base < nj; base += 256) {
+      const int j = base + tid;
+      int u = 0;
+      if (j < nj) {
+        const JobState* s = a.jobs[j].state;
+        u = s->done ? 0 : s->units;
       }
+      int tot;
+      const int ex = block_exclusive_scan<256>(u, &tot, s_part);
+      if (j < nj) s_pre[j] = carry + ex;
+      carry += tot;
     }
-    uint64_t surv = __builtin_amdgcn_ballot_w64(ok);
-    while (surv) {
-      const int bit = __builtin_ctzll(surv);
-      surv &= surv - 1;
-      const int q2 = __shfl(q, bit);
-      const int m2 = __shfl(mq, bit);
-      const int b2 = __shfl(bcq, bit);
-      const int64_t e0 = a.st.kpoff[q2], e1 = a.st.kpoff[q2 + 1];
-      int acc = 0;
-      for (int64_t e = e0 + lane; e < e1; e += 64) {
-        const uint32_t x = a.st.kp[e];
-        acc += min((uint32_t)table[x & 0xFFFFu], x >> 16);
+    if (tid == 0) s_pre[nj] = carry;
+    __syncthreads();
+    total = s_pre[nj];
+  }
+  if (total == 0) return;
+  const int per = (total + gridDim.x - 1) / gridDim.x;
+  const int u0 = blockIdx.x * per;
+  const int u1 = min(total, u0 + per);
+  const int g = tid >> 4, gl = tid & 15;
+  int cj = -1, cs = -1;
+  unsigned long long rej_len = 0, rej_comp = 0, rej_word = 0, passed = 0, bytes = 0, cband = 0, cfull = 0;
+  for (int u = u0; u < u1; ++u) {
+    // job of unit u: last j with s_pre[j] <= u
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    const int job = lo;
+    const JobDev& J = a.jobs[job];
+    const JobState* S = J.state;
+    const int nc = S->nc;
+    const int32_t* cand = J.cand[S->cur];
+    const int chunks = (nc + kFilterChunk - 1) / kFilterChunk;
+    const int lu = u - s_pre[job];
+    const int slot = lu / chunks;
+    const int chunk = lu - slot * chunks;
+    const int r = cand[slot];
+    if (job != cj || slot != cs) {
+      const int nw = 1 << (2 * J.k);
+      __syncthreads();
+      for (int i = tid; i < nw / 2; i += 256) lds[i] = 0u;
+      __syncthreads();
+      const int64_t k0 = J.st.kpoff[r], k1 = J.st.kpoff[r + 1];
+      for (int64_t e = k0 + tid; e < k1; e += 256) {
+        const uint32_t x = J.st.kp[e];
+        table[x & 0xFFFFu] = (uint16_t)(x >> 16);
       }
-      acc = wave_sum(acc);
-      if (lane == 0) {
-        bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
-        if (acc < a.st.wc[q2]) {
-          ++rej_word;
-        } else {
-          ++passed;
-          bytes += 8;
-          cband += (unsigned long long)band_cells(n, m2, b2 + 1);
-          cfull += (unsigned long long)n * (unsigned long long)m2;
-          const int sh = pick_shape(n, m2, b2 + 1, a.min_shape);
-          const int pos = atomicAdd(&a.bin_count[sh], 1);
-          a.items[(int64_t)sh * a.item_cap + pos] = make_int2(r, q2);
-        }
+      __syncthreads();
+      cj = job;
+      cs = slot;
+    }
+    const int pos = chunk * kFilterChunk + g;
+    if (pos <= slot || pos >= nc) continue;
+    const int q = cand[pos];
+    const int n = J.st.m[r];
+    const int mq = J.st.m[q];
+    const int bcq = J.st.bc[q];
+    const int c = bcq + 1;
+    if (gl == 0) bytes += 28;  // id, m, bc, comp
+    if (c > min(n, mq)) {
+      if (gl == 0) ++rej_len;
+      continue;
+    }
+    const uint4 cr = J.st.comp[r], cq = J.st.comp[q];
+    const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
+    if (sum <= bcq) {
+      if (gl == 0) ++rej_comp;
+      continue;
+    }
+    const int64_t e0 = J.st.kpoff[q], e1 = J.st.kpoff[q + 1];
+    int acc = 0;
+    int64_t e = e0 + gl;
+    for (; e + 48 < e1; e += 64) {
+      const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
+      acc += min((uint32_t)table[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)table[x1 & 0xFFFFu], x1 >> 16) +
+             min((uint32_t)table[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)table[x3 & 0xFFFFu], x3 >> 16);
+    }
+    for (; e < e1; e += 16) {
+      const uint32_t x = J.st.kp[e];
+      acc += min((uint32_t)table[x & 0xFFFFu], x >> 16);
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (gl == 0) {
+      bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
+      if (acc < J.st.wc[q]) {
+        ++rej_word;
+      } else {
+        ++passed;
+        bytes += 16;
+        cband += (unsigned long long)band_cells(n, mq, c);
+        cfull += (unsigned long long)n * (unsigned long long)mq;
+        const int sh = pick_shape(n, mq, c, a.min_shape);
+        const int at = atomicAdd(&a.bin_count[sh], 1);
+        AlignItem it;
+        it.text = r;
+        it.query = q;
+        it.job = job;
+        it.slot = slot;
+        a.items[(int64_t)sh * a.item_cap + at] = it;
       }
     }
   }
-  // per-wave counters
   unsigned long long v[7] = {rej_len, rej_comp, rej_word, passed, bytes, cband, cfull};
+  const int lane = tid & 63;
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
     unsigned long long x = v[i];
@@ -133,18 +218,15 @@ __device__ inline void filter_wave_loop(const FilterArgs& a, const uint16_t* tab
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     v[i] = x;
   }
-  if (lane == 0) {
-    const unsigned long long pairs = v[0] + v[1] + v[2] + v[3];
-    if (pairs) {
-      atomicAdd(&a.stats[ST_PAIRS], pairs);
-      if (v[0]) atomicAdd(&a.stats[ST_REJ_LEN], v[0]);
-      if (v[1]) atomicAdd(&a.stats[ST_REJ_COMP], v[1]);
-      if (v[2]) atomicAdd(&a.stats[ST_REJ_WORD], v[2]);
-      if (v[3]) atomicAdd(&a.stats[ST_ALIGNED], v[3]);
-      atomicAdd(&a.stats[ST_FILTER_BYTES], v[4]);
-      if (v[5]) atomicAdd(&a.stats[ST_CELLS_BAND], v[5]);
-      if (v[6]) atomicAdd(&a.stats[ST_CELLS_FULL], v[6]);
-    }
+  if (lane == 0 && (v[0] | v[1] | v[2] | v[3])) {
+    atomicAdd(&a.stats[ST_PAIRS], v[0] + v[1] + v[2] + v[3]);
+    if (v[0]) atomicAdd(&a.stats[ST_REJ_LEN], v[0]);
+    if (v[1]) atomicAdd(&a.stats[ST_REJ_COMP], v[1]);
+    if (v[2]) atomicAdd(&a.stats[ST_REJ_WORD], v[2]);
+    if (v[3]) atomicAdd(&a.stats[ST_ALIGNED], v[3]);
+    atomicAdd(&a.stats[ST_FILTER_BYTES], v[4]);
+    if (v[5]) atomicAdd(&a.stats[ST_CELLS_BAND], v[5]);
+    if (v[6]) atomicAdd(&a.stats[ST_CELLS_FULL], v[6]);
   }
 }
 
@@ -195,10 +277,12 @@ __device__ inline void align_loop(const AlignArgs& a) {
 
   int item = -1;
   bool exhausted = false;
-  int tx = 0, q = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
+  int tx = 0, q = 0, job = 0, slot = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
+  const uint2* pk = nullptr;
   int64_t toff = 0, qoff = 0;
   unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0;
   const int total = *a.count;
+  if (total == 0) return;
 
   while (true) {
     // ---- fetch a pair for every segment without one
@@ -209,14 +293,18 @@ __device__ inline void align_loop(const AlignArgs& a) {
     if (need) {
       if (got < total) {
         item = got;
-        const int2 it = a.items[got];
-        tx = it.x;
-        q = it.y;
-        n = a.st.m[tx];
-        mq = a.st.m[q];
-        c = a.st.bc[q] + 1;
-        toff = a.st.pkoff[tx];
-        qoff = a.st.pkoff[q];
+        const AlignItem it = a.items[got];
+        tx = it.text;
+        q = it.query;
+        job = it.job;
+        slot = it.slot;
+        const StoreView& st = a.jobs[job].st;
+        pk = st.pk;
+        n = st.m[tx];
+        mq = st.m[q];
+        c = st.bc[q] + 1;
+        toff = st.pkoff[tx];
+        qoff = st.pkoff[q];
         ntw = (n >> 5) + 1;
         b = 0;
         jb = 0;
@@ -225,7 +313,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
         for (int w = 0; w < W; ++w) {
           const int gw = sl * W + w;
           uint2 t = make_uint2(0u, 0u);
-          if (gw < ntw) t = a.st.pk[toff + gw];
+          if (gw < ntw) t = pk[toff + gw];
           tl[w] = t.x;
           th[w] = t.y;
           v[w] = ~0u;
@@ -261,7 +349,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
           for (int w = 0; w < W; ++w) {
             const int gw = b + S * W + w;
             uint2 t = make_uint2(0u, 0u);
-            if (gw < ntw) t = a.st.pk[toff + gw];
+            if (gw < ntw) t = pk[toff + gw];
             tl[w] = t.x;
             th[w] = t.y;
             v[w] = ~0u;
@@ -274,7 +362,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
     // ---- query word for this 32-row block
     uint32_t qlo = 0u, qhi = 0u, qnv = ~0u;
     if (active && !hopeless) {
-      const uint2 qw = a.st.pk[qoff + jb];
+      const uint2 qw = pk[qoff + jb];
       qlo = qw.x;
       qhi = qw.y;
       const int rows = mq - jb * 32;
@@ -284,36 +372,36 @@ __device__ inline void align_loop(const AlignArgs& a) {
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
     for (int r0 = 0; r0 < 32; r0 += UR) {
-     const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
+      const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
 #pragma unroll
-     for (int r = 0; r < UR; ++r) {
-      const uint32_t ql = 0u - ((qlc >> r) & 1u);
-      const uint32_t qh = 0u - ((qhc >> r) & 1u);
-      const uint32_t qn = 0u - ((qnc >> r) & 1u);
-      uint32_t x[W], s[W];
-      uint32_t carry = 0u, pall = ~0u;
+      for (int r = 0; r < UR; ++r) {
+        const uint32_t ql = 0u - ((qlc >> r) & 1u);
+        const uint32_t qh = 0u - ((qhc >> r) & 1u);
+        const uint32_t qn = 0u - ((qnc >> r) & 1u);
+        uint32_t x[W], s[W];
+        uint32_t carry = 0u, pall = ~0u;
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        x[w] = (tl[w] ^ ql) | (th[w] ^ qh) | qn;
-        const uint32_t u = v[w] & ~x[w];
-        uint32_t co;
-        s[w] = __builtin_addc(v[w], u, carry, &co);
-        carry = co;
-        pall &= s[w];
+        for (int w = 0; w < W; ++w) {
+          x[w] = (tl[w] ^ ql) | (th[w] ^ qh) | qn;
+          const uint32_t u = v[w] & ~x[w];
+          uint32_t co;
+          s[w] = __builtin_addc(v[w], u, carry, &co);
+          carry = co;
+          pall &= s[w];
+        }
+        const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
+        const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+        const uint64_t X = G | P;
+        const uint64_t C = (X + G) ^ X ^ G;
+        uint32_t cin = (uint32_t)(C >> lane) & 1u;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          uint32_t co;
+          const uint32_t t = __builtin_addc(s[w], 0u, cin, &co);
+          cin = co;
+          v[w] = t | (v[w] & x[w]);
+        }
       }
-      const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
-      const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
-      const uint64_t X = G | P;
-      const uint64_t C = (X + G) ^ X ^ G;
-      uint32_t cin = (uint32_t)(C >> lane) & 1u;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        uint32_t co;
-        const uint32_t t = __builtin_addc(s[w], 0u, cin, &co);
-        cin = co;
-        v[w] = t | (v[w] & x[w]);
-      }
-     }
     }
 
     // ---- block end: exact early decisions on the zero count
@@ -337,8 +425,11 @@ __device__ inline void align_loop(const AlignArgs& a) {
       if (accept || reject) {
         if (sl == 0) {
           if (!hopeless) cdec += (unsigned long long)band_cells_rows(n, mq, c, jb * 32 + 32);
-          if (accept && a.cluster) a.cluster[q] = tx;
-          if (a.result) a.result[item] = accept ? 1 : 0;
+          if (a.result) {
+            a.result[item] = accept ? 1 : 0;
+          } else if (accept) {
+            atomicOr(&a.jobs[job].acc[q], 1ull << slot);
+          }
           if (accept) ++nacc; else ++nrej;
         }
         item = -1;
@@ -368,11 +459,13 @@ __global__ __launch_bounds__(256) void align_kernel(AlignArgs a) {
   align_loop<S, W>(a);
 }
 
+#endif  // GSDR_DEVICE_TU
+
 // host launchers (kernels.hip)
-hipError_t launch_filter(int k, const FilterArgs& a, int grid, hipStream_t s);
+hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s);
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
-hipError_t launch_compact(const int32_t* cand, RoundState* st, int32_t* cluster, int32_t* bcount,
-                          int32_t* out, int nc, int32_t* reset, int nreset, hipStream_t s);
-hipError_t launch_init(int32_t* cluster, int32_t* cand, int n, RoundState* st, hipStream_t s);
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
+hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
+                            int32_t* jobs_left, hipStream_t s);
 
 }  // namespace gsdr

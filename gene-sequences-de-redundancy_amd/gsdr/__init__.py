@@ -247,7 +247,9 @@ class Engine:
     def load(self, sorted_records, word_length):
         h = C.c_void_p()
         _check(lib().gsdr_load(self.h, sorted_records.h, word_length, C.byref(h)), "load")
-        return Store(self, h, len(sorted_records))
+        st = Store(self, h, len(sorted_records))
+        st.k = word_length
+        return st
 
     def align_pairs(self, datas, pairs, bc, shape=-1):
         """LCS(text, query) > bc per (text, query) pair, decided on the device."""

@@ -83,3 +83,53 @@ def test_single_record_and_ties(engine, tmp_path):
     assert np.array_equal(r2.sort(), operm)
     cl2, _ = gsdr.cluster_records(engine, r2, 0.95)
     assert np.array_equal(cl2, ocl)
+
+
+@pytest.mark.parametrize("config,n,t", [(1, 10000, 0.95), (2, 6000, 0.90), (4, 6000, 0.95), (4, 6000, 0.80),
+                                        (2, 3000, 0.99)])
+def test_batched_matches_oracle(engine, tmp_path, config, n, t):
+    recs = gsdr.Records.generate(config, 2000 + config, n)
+    cl, ocl, st, ost, same_file = run_both(engine, tmp_path, recs, t, mode=gsdr.MODE_BATCHED)
+    bad = np.nonzero(cl != ocl)[0]
+    assert len(bad) == 0, f"{len(bad)} assignments differ, first at {bad[:5]}"
+    assert same_file
+    assert st.reps == ost["reps"]
+    assert st.rounds < ost["reps"] or ost["reps"] < 8  # speculation fixes several representatives per step
+
+
+@pytest.mark.parametrize("spec", [1, 2, 7, 64])
+def test_speculation_width_invariant(engine, spec):
+    recs = gsdr.Records.generate(2, 77, 5000)
+    recs.sort()
+    store = engine.load(recs, 5)
+    try:
+        base, _ = store.cluster(0.9, 5, mode=gsdr.MODE_PLAIN)
+        got, st = store.cluster(0.9, 5, mode=gsdr.MODE_BATCHED, spec_reps=spec)
+        assert np.array_equal(base, got)
+        timed, _ = store.cluster(0.9, 5, mode=gsdr.MODE_BATCHED, spec_reps=spec, timing=True)
+        assert np.array_equal(base, timed)
+    finally:
+        store.free()
+
+
+def test_jobs_alone_equals_in_mix(engine):
+    specs = [(2, 11, 3000, 0.90), (1, 12, 2500, 0.95), (4, 13, 4000, 0.80), (2, 14, 1500, 0.85), (4, 15, 3000, 0.99),
+             (1, 16, 800, 0.88)]
+    stores, alone, ts = [], [], []
+    try:
+        for cfg, seed, n, t in specs:
+            r = gsdr.Records.generate(cfg, seed, n)
+            r.sort()
+            k = gsdr.auto_word_length(t)
+            s = engine.load(r, k)
+            stores.append(s)
+            ts.append(t)
+            cl, _ = s.cluster(t, k, mode=gsdr.MODE_PLAIN)
+            alone.append(cl)
+        mixed, st = engine.cluster_jobs(stores, ts, mode=gsdr.MODE_BATCHED)
+        for a, b in zip(alone, mixed):
+            assert np.array_equal(a, b)
+        assert st.n_seqs == sum(s.n for s in stores)
+    finally:
+        for s in stores:
+            s.free()
