@@ -132,7 +132,7 @@ def main():
         "rounds": int(st.rounds),
         "stage": {"pairs": st.pairs, "rej_length": st.rej_length, "rej_comp": st.rej_comp,
                   "rej_word": st.rej_word, "aligned": st.aligned, "accepted": st.accepted,
-                  "rej_align": st.rej_align},
+                  "rej_align": st.rej_align, "align_skipped": int(st.spec_wasted)},
         "cells": {"band": st.cells_band, "decided": st.cells_decided, "exec": st.cells_exec,
                   "full": st.cells_full},
         "kernel_ms": {"align": round(st.ms_align, 3), "filter": round(st.ms_filter, 3),

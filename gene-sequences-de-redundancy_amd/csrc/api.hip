@@ -173,6 +173,11 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * 4, hipHostMallocDefault), "hipHostMalloc");
   c->h_state_cap = 4;
+  for (int i = 0; i < kNumShapes; ++i) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking), "hipStreamCreate");
+    HIP_TRY(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming), "hipEventCreate");
+  }
+  HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate");
   *out = c.release();
   return GSDR_OK;
 }
@@ -181,6 +186,11 @@ void gsdr_close(gsdr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   workspace_release(c);
+  for (int i = 0; i < kNumShapes; ++i) {
+    if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    if (c->join[i]) (void)hipEventDestroy(c->join[i]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->h_state) (void)hipHostFree(c->h_state);
   delete c;

@@ -22,24 +22,36 @@ namespace gsdr {
 namespace {
 
 struct StepLaunch {
-  FilterArgs fa;
+  FilterArgs fa_reps, fa_main;
   ResolveArgs ra;
+  AlignArgs aa_reps[kNumShapes];
   AlignArgs aa[kNumShapes];
   int smax = 0;
   int max_k = 4;
   int filter_grid = 1;
-  int align_grid = 1;
+  int align_grid[kNumShapes] = {};
+  int reps_grid[kNumShapes] = {};
+  int reps_filter_grid = 1;
 };
 
-// One step for every job: filter, alignment per tile shape, resolve.
-hipError_t launch_step(const StepLaunch& L, hipStream_t q, hipEvent_t* ev) {
+// One step for every job. Phase A: the speculative representatives against
+// each other, then resolve_reps fixes the true ones. Phase B: filter,
+// alignment (skipping candidates an earlier true representative already
+// took), resolve. The tile-shape kernels of a phase run concurrently.
+// ev (optional) brackets filter and alignment time of both phases.
+hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev) {
   hipError_t e;
   if (ev && (e = hipEventRecord(ev[0], q)) != hipSuccess) return e;
-  if ((e = launch_filter(L.fa, L.max_k, L.filter_grid, q)) != hipSuccess) return e;
+  if ((e = launch_filter(L.fa_reps, L.max_k, L.reps_filter_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
-  for (int sh = 0; sh <= L.smax; ++sh)
-    if ((e = launch_align(sh, L.aa[sh], L.align_grid, q)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
+  if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
+  if ((e = launch_filter(L.fa_main, L.max_k, L.filter_grid, q)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
   return launch_resolve(L.ra, q);
 }
 
@@ -95,14 +107,23 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   L.max_k = max_k;
   L.smax = smax;
   L.filter_grid = c->cu_count * 8;
-  L.align_grid = c->cu_count * 4;
-  L.fa.jobs = w.jobs;
-  L.fa.njobs = njobs;
-  L.fa.min_shape = 0;
-  L.fa.items = w.items;
-  L.fa.item_cap = w.item_cap;
-  L.fa.bin_count = w.counters;
-  L.fa.stats = w.stats;
+  L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    L.align_grid[sh] = c->cu_count * 4;
+    // phase A holds at most k_spec^2/2 pairs per job
+    const int segs = 64 / kShapes[sh].s;
+    L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));
+  }
+  L.fa_main.jobs = w.jobs;
+  L.fa_main.njobs = njobs;
+  L.fa_main.min_shape = 0;
+  L.fa_main.phase = PHASE_MAIN;
+  L.fa_main.items = w.items;
+  L.fa_main.item_cap = w.item_cap;
+  L.fa_main.bin_count = w.counters;
+  L.fa_main.stats = w.stats;
+  L.fa_reps = L.fa_main;
+  L.fa_reps.phase = PHASE_REPS;
   for (int sh = 0; sh < kNumShapes; ++sh) {
     AlignArgs& aa = L.aa[sh];
     aa.jobs = w.jobs;
@@ -111,6 +132,11 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     aa.work = w.counters + kNumShapes + sh;
     aa.result = nullptr;
     aa.stats = w.stats;
+    aa.skip_absorbed = 1;
+  }
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    L.aa_reps[sh] = L.aa[sh];
+    L.aa_reps[sh].skip_absorbed = 0;
   }
   L.ra.jobs = w.jobs;
   L.ra.njobs = njobs;
@@ -127,19 +153,21 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   if (timing) {
     // timed form: the same step launched directly with events around the
     // filter and the alignment kernels
-    hipEvent_t ev[3];
+    hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
     while (true) {
-      HIP_TRY(launch_step(L, q, ev), "step launch");
-      launches += 2 + L.smax + 1;
+      HIP_TRY(launch_step(c, L, q, ev), "step launch");
+      launches += 3 + 2 * (L.smax + 1) + 1;
       ++steps_launched;
       HIP_TRY(hipMemcpyAsync(c->h_state, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
       HIP_TRY(hipStreamSynchronize(q), "step sync");
-      float a = 0, b = 0;
-      HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
-      HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
-      ms_filter += a;
-      ms_align += b;
+      float t01 = 0, t12 = 0, t34 = 0, t45 = 0;
+      HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]), "event time");
+      HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]), "event time");
+      HIP_TRY(hipEventElapsedTime(&t34, ev[3], ev[4]), "event time");
+      HIP_TRY(hipEventElapsedTime(&t45, ev[4], ev[5]), "event time");
+      ms_filter += t01 + t34;
+      ms_align += t12 + t45;
       if (*reinterpret_cast<int32_t*>(c->h_state) == 0) break;
       if (steps_launched > step_limit) return fail(GSDR_E_DEVICE, "steps did not converge");
     }
@@ -151,7 +179,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     hipGraphExec_t exec = nullptr;
     HIP_TRY(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int b = 0; b < B; ++b) {
-      hipError_t e = launch_step(L, q, nullptr);
+      hipError_t e = launch_step(c, L, q, nullptr);
       if (e != hipSuccess) {
         hipGraph_t g2;
         (void)hipStreamEndCapture(q, &g2);
@@ -170,7 +198,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       if (e != hipSuccess) { rc = hip_fail(e, "graph replay"); break; }
       ++replays;
       steps_launched += B;
-      launches += (int64_t)B * (2 + L.smax + 1);
+      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1);
       if (*h_left == 0) break;
       if (steps_launched > step_limit + B) { rc = fail(GSDR_E_DEVICE, "steps did not converge"); break; }
     }
@@ -204,8 +232,8 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     stats->ms_total = ms;
     stats->ms_filter = ms_filter;
     stats->ms_align = ms_align;
-    stats->filter_launches = steps_launched;
-    stats->align_launches = steps_launched * (L.smax + 1);
+    stats->filter_launches = 2 * steps_launched;
+    stats->align_launches = steps_launched * 2 * (L.smax + 1);
   }
   return GSDR_OK;
 }

@@ -159,6 +159,19 @@ int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   return cluster_plain(c, s, p, out, stats);
 }
 
+hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main) {
+  hipError_t e;
+  if ((e = hipEventRecord(c->fork, main)) != hipSuccess) return e;
+  for (int sh = 0; sh <= smax; ++sh) {
+    if ((e = hipStreamWaitEvent(c->side[sh], c->fork, 0)) != hipSuccess) return e;
+    if ((e = launch_align(sh, args[sh], grids[sh], c->side[sh])) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->join[sh], c->side[sh])) != hipSuccess) return e;
+  }
+  for (int sh = 0; sh <= smax; ++sh)
+    if ((e = hipStreamWaitEvent(main, c->join[sh], 0)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
 // Largest tile shape a representative of gap-free length n can need.
 int max_shape_for(int n) {
   const int ntw = (n >> 5) + 1;
@@ -205,6 +218,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     fa.jobs = w.jobs;
     fa.njobs = 1;
     fa.min_shape = 0;
+    fa.phase = PHASE_MAIN;
     fa.items = w.items;
     fa.item_cap = w.item_cap;
     fa.bin_count = w.counters;
@@ -214,20 +228,23 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     ++filter_launches;
     if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
     const int smax = max_shape_for(s.m[first]);
+    AlignArgs aa[kNumShapes];
+    int grids[kNumShapes];
     for (int sh = 0; sh <= smax; ++sh) {
-      AlignArgs aa;
-      aa.jobs = w.jobs;
-      aa.items = w.items + (int64_t)sh * w.item_cap;
-      aa.count = w.counters + sh;
-      aa.work = w.counters + kNumShapes + sh;
-      aa.result = nullptr;
-      aa.stats = w.stats;
+      aa[sh].jobs = w.jobs;
+      aa[sh].items = w.items + (int64_t)sh * w.item_cap;
+      aa[sh].count = w.counters + sh;
+      aa[sh].work = w.counters + kNumShapes + sh;
+      aa[sh].result = nullptr;
+      aa[sh].stats = w.stats;
+      aa[sh].skip_absorbed = 0;
       const int segs = 64 / kShapes[sh].s;
       const int waves = (nc + segs - 1) / segs;
-      HIP_TRY(launch_align(sh, aa, std::max(1, std::min((waves + 3) / 4, cap_blocks)), q), "align launch");
+      grids[sh] = std::max(1, std::min((waves + 3) / 4, cap_blocks));
       ++launches;
       ++align_launches;
     }
+    HIP_TRY(launch_align_shapes(c, aa, smax, grids, q), "align launch");
     if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
     ResolveArgs ra;
     ra.jobs = w.jobs;
@@ -377,6 +394,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     aa.work = ctr + kNumShapes + sh;
     aa.result = res + first[sh];
     aa.stats = d_stats;
+    aa.skip_absorbed = 0;
     const int segs = 64 / kShapes[sh].s;
     const int waves = (counts[sh] + segs - 1) / segs;
     const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * 4));

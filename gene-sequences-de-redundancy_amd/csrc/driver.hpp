@@ -38,6 +38,11 @@ struct gsdr_ctx {
   int32_t h_state_cap = 0;
   int32_t* h_left = nullptr;          // pinned copy of jobs_left
   Workspace ws;
+  // side streams (default priority) that run the tile-shape kernels of a
+  // step concurrently, with fork/join events
+  hipStream_t side[gsdr::kNumShapes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[gsdr::kNumShapes] = {};
 };
 
 struct gsdr_store {
@@ -96,6 +101,10 @@ int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out);
 int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs);
 void workspace_release(gsdr_ctx* c);
 int max_shape_for(int n);
+struct AlignArgs;
+// Launches the alignment kernels of shapes 0..smax concurrently on the side
+// streams, forked from and joined back into `main`.
+hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main);
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
 int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);

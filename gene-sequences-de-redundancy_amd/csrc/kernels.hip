@@ -23,11 +23,43 @@ namespace gsdr {
 // The unassigned candidates are then compacted in order (updateRemain,
 // func.cpp:477-487), so the next step's representatives are again the
 // smallest unclustered indices (updateRepresentative, func.cpp:466-476).
+//
+// The step runs in two phases. Phase A aligns the speculative
+// representatives against each other only; resolve_reps_kernel then fixes
+// which slots are true representatives (tmask). Phase B aligns the true
+// representatives against the remaining candidates and resolve_kernel
+// assigns and compacts them.
+__global__ __launch_bounds__(64) void resolve_reps_kernel(ResolveArgs a) {
+  const int tid = threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int i = tid; i < a.ncounters; i += 64) a.counters[i] = 0;
+  if (tid != 0) return;
+  const JobDev& J = a.jobs[blockIdx.x];
+  JobState* S = J.state;
+  if (S->done) return;
+  const int K = S->k_spec;
+  const int32_t* cand = J.cand[S->cur];
+  unsigned long long tm = 0;
+  int nonrep = 0;
+  for (int i = 0; i < K; ++i) {
+    const int r = cand[i];
+    const unsigned long long msk = J.acc[r] & tm;
+    if (msk == 0) {
+      tm |= 1ull << i;
+      J.cluster[r] = r;
+    } else {
+      J.cluster[r] = cand[__builtin_ctzll(msk)];
+      ++nonrep;
+    }
+    J.acc[r] = 0;
+  }
+  S->tmask = tm;
+  S->kt = __popcll(tm);
+  S->pad[0] = nonrep;
+}
+
 __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
   __shared__ int s_part[16];
-  __shared__ unsigned long long s_tm;
-  __shared__ int s_nonrep;
-  __shared__ long long s_waste;
   const int tid = threadIdx.x;
   if (blockIdx.x == 0)
     for (int i = tid; i < a.ncounters; i += 1024) a.counters[i] = 0;
@@ -37,29 +69,8 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
   const int K = S->k_spec, nc = S->nc, cur = S->cur;
   const int32_t* cand = J.cand[cur];
   int32_t* out = J.cand[cur ^ 1];
-  if (tid == 0) {
-    unsigned long long tm = 0;
-    int nonrep = 0;
-    long long waste = 0;
-    for (int i = 0; i < K; ++i) {
-      const int r = cand[i];
-      const unsigned long long msk = J.acc[r] & tm;
-      if (msk == 0) {
-        tm |= 1ull << i;
-        J.cluster[r] = r;
-      } else {
-        J.cluster[r] = cand[__builtin_ctzll(msk)];
-        ++nonrep;
-        waste += nc - i - 1;
-      }
-      J.acc[r] = 0;
-    }
-    s_tm = tm;
-    s_nonrep = nonrep;
-    s_waste = waste;
-  }
-  __syncthreads();
-  const unsigned long long tm = s_tm;
+  const unsigned long long tm = S->tmask;
+  if (tid < K && ((tm >> tid) & 1ull)) J.cluster[cand[tid]] = cand[tid];
   int absorbed = 0;
   int written = 0;
   for (int base = K; base < nc; base += 4096) {
@@ -104,15 +115,17 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
     S->nc = nc2;
     int k2 = 1;
     if (S->k_max > 1) {
-      // absorption per true representative and candidate; a step of K
-      // speculative representatives wastes about K*f/2 of its pair work
-      const double denom = (double)true_reps * (double)max(nc - K, 1);
-      const double f = (double)(absorbed + s_nonrep) / denom;
+      // f = absorption per true representative and candidate. A step of K
+      // slots evaluates about K*f/2 pairs per candidate whose candidate an
+      // earlier slot already took (fewer where the align fetch skips them),
+      // and phase A costs K^2/2 pairs: aim at K*f near 1/2.
+      const double denom = (double)max(true_reps, 1) * (double)max(nc - K, 1);
+      const double f = (double)absorbed / denom;
       const int up = min(2 * K, S->k_max), down = max(1, K / 2);
       if (f <= 0.0) {
         k2 = up;
       } else {
-        const double want = 0.25 / f;
+        const double want = 0.5 / f;
         k2 = want >= up ? up : (want <= down ? down : (int)want);
       }
     }
@@ -123,13 +136,15 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
     if (k2 > nc2) k2 = nc2;
     if (k2 < 1) k2 = 1;
     S->k_spec = k2;
-    S->units = nc2 > 0 ? k2 * ((nc2 + kFilterChunk - 1) / kFilterChunk) : 0;
+    // with one slot, phase A has nothing to decide: slot 0 is the
+    // representative (updateRepresentative, func.cpp:466-476)
+    S->tmask = 1ull;
+    S->kt = 1;
     S->first = nc2 > 0 ? out[0] : -1;
     if (nc2 == 0) {
       S->done = 1;
       if (a.jobs_left) atomicSub(a.jobs_left, 1);
     }
-    if (s_waste) atomicAdd(&a.stats[ST_SPEC_WASTE], (unsigned long long)s_waste);
   }
 }
 
@@ -150,7 +165,10 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
     S->nc = n;
     S->k_spec = 1;
     S->done = n > 0 ? 0 : 1;
-    S->units = n > 0 ? (n + kFilterChunk - 1) / kFilterChunk : 0;
+    S->kt = 1;
+    S->tmask = 1ull;
+    S->pad[0] = S->pad[1] = S->pad[2] = 0;
+    S->pad2 = 0ull;
     S->steps = 0;
     S->reps = 0;
     S->k_max = k_max;
@@ -190,6 +208,11 @@ hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s) 
     case 7: return launch_align_sw<64, 16>(a, grid, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(resolve_reps_kernel, dim3(a.njobs), dim3(64), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s) {

@@ -90,13 +90,28 @@ struct JobState {
   int32_t nc;       // entries in the current list
   int32_t k_spec;   // speculative representatives this step
   int32_t done;     // 1 when every sequence has a cluster
-  int32_t units;    // filter work units of this step: k_spec * ceil(nc / kFilterChunk)
   int32_t steps;    // steps taken
   int32_t reps;     // representatives fixed so far
   int32_t k_max;    // upper bound for k_spec (1 = plain form)
   int32_t first;    // cand[cur][0], the next representative (-1 when done)
+  int32_t kt;       // true representatives among the k_spec slots (after phase A)
   int32_t pad[3];
+  unsigned long long tmask;  // slots that are true representatives (after phase A)
+  unsigned long long pad2;
 };
+
+// Filter phases of a step: A = speculative representatives against each
+// other (slots 0..k_spec-1), B = true representatives against the rest.
+enum FilterPhase { PHASE_REPS = 0, PHASE_MAIN = 1 };
+
+// Work units of a job in a phase (kFilterChunk candidate positions each).
+__host__ __device__ inline int phase_units(const JobState& s, int phase) {
+  if (s.done) return 0;
+  const int K = s.k_spec;
+  if (phase == PHASE_REPS) return K > 1 ? K * ((K + kFilterChunk - 1) / kFilterChunk) : 0;
+  const int rest = s.nc - K;
+  return rest > 0 ? s.kt * ((rest + kFilterChunk - 1) / kFilterChunk) : 0;
+}
 
 // One clustering job as the kernels see it.
 struct JobDev {

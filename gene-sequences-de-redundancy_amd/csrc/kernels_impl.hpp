@@ -10,6 +10,7 @@ struct FilterArgs {
   const JobDev* jobs;
   int32_t njobs;
   int32_t min_shape;
+  int32_t phase;            // FilterPhase
   AlignItem* items;         // per-shape lists: items + shape * item_cap
   int64_t item_cap;
   int32_t* bin_count;       // kNumShapes counters
@@ -23,6 +24,7 @@ struct AlignArgs {
   int32_t* work;            // work counter (device, starts at 0)
   int32_t* result;          // non-null: result[item] = 1 accept / 0 reject (pair-level entry)
   unsigned long long* stats;
+  int32_t skip_absorbed;    // skip (slot, q) when an earlier true representative already accepted q
 };
 
 struct ResolveArgs {
@@ -106,10 +108,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     for (int base = 0; base < nj; base += 256) {
       const int j = base + tid;
       int u = 0;
-      if (j < nj) {
-        const JobState* s = a.jobs[j].state;
-        u = s->done ? 0 : s->units;
-      }
+      if (j < nj) u = phase_units(*a.jobs[j].state, a.phase);
       int tot;
       const int ex = block_exclusive_scan<256>(u, &tot, s_part);
       if (j < nj) s_pre[j] = carry + ex;
@@ -137,11 +136,22 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     const JobDev& J = a.jobs[job];
     const JobState* S = J.state;
     const int nc = S->nc;
+    const int K = S->k_spec;
     const int32_t* cand = J.cand[S->cur];
-    const int chunks = (nc + kFilterChunk - 1) / kFilterChunk;
+    // phase A: slots 0..K-1 against positions 0..K-1; phase B: the rank-th
+    // true representative against positions K..nc-1
+    const int p0 = a.phase == PHASE_REPS ? 0 : K;
+    const int p1 = a.phase == PHASE_REPS ? K : nc;
+    const int chunks = (p1 - p0 + kFilterChunk - 1) / kFilterChunk;
     const int lu = u - s_pre[job];
-    const int slot = lu / chunks;
-    const int chunk = lu - slot * chunks;
+    const int rank = lu / chunks;
+    const int chunk = lu - rank * chunks;
+    int slot = rank;
+    if (a.phase == PHASE_MAIN) {
+      unsigned long long tm = S->tmask;
+      for (int i = 0; i < rank; ++i) tm &= tm - 1;
+      slot = __builtin_ctzll(tm);
+    }
     const int r = cand[slot];
     if (job != cj || slot != cs) {
       const int nw = 1 << (2 * J.k);
@@ -157,8 +167,8 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       cj = job;
       cs = slot;
     }
-    const int pos = chunk * kFilterChunk + g;
-    if (pos <= slot || pos >= nc) continue;
+    const int pos = p0 + chunk * kFilterChunk + g;
+    if (pos <= slot || pos >= p1) continue;
     const int q = cand[pos];
     const int n = J.st.m[r];
     const int mq = J.st.m[q];
@@ -280,52 +290,68 @@ __device__ inline void align_loop(const AlignArgs& a) {
   int tx = 0, q = 0, job = 0, slot = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
   const uint2* pk = nullptr;
   int64_t toff = 0, qoff = 0;
-  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0;
+  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
   const int total = *a.count;
   if (total == 0) return;
 
   while (true) {
-    // ---- fetch a pair for every segment without one
-    const bool need = (item < 0) && !exhausted;
-    int got = 0;
-    if (need && sl == 0) got = atomicAdd(a.work, 1);
-    got = __shfl(got, seg0);
-    if (need) {
-      if (got < total) {
-        item = got;
-        const AlignItem it = a.items[got];
-        tx = it.text;
-        q = it.query;
-        job = it.job;
-        slot = it.slot;
-        const StoreView& st = a.jobs[job].st;
-        pk = st.pk;
-        n = st.m[tx];
-        mq = st.m[q];
-        c = st.bc[q] + 1;
-        toff = st.pkoff[tx];
-        qoff = st.pkoff[q];
-        ntw = (n >> 5) + 1;
-        b = 0;
-        jb = 0;
-        frozen = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          const int gw = sl * W + w;
-          uint2 t = make_uint2(0u, 0u);
-          if (gw < ntw) t = pk[toff + gw];
-          tl[w] = t.x;
-          th[w] = t.y;
-          v[w] = ~0u;
-        }
-      } else {
+    // ---- fetch a pair for every segment without one; pairs decided without
+    // alignment (LCS <= min(n, m) < c, or q already taken by an earlier true
+    // representative of this step) are settled here and the fetch repeats
+    while (true) {
+      const bool need = (item < 0) && !exhausted;
+      if (__builtin_amdgcn_ballot_w64(need) == 0) break;
+      int got = 0;
+      if (need && sl == 0) got = atomicAdd(a.work, 1);
+      got = __shfl(got, seg0);
+      if (!need) continue;
+      if (got >= total) {
         exhausted = true;
+        continue;
+      }
+      const AlignItem it = a.items[got];
+      const JobDev& J = a.jobs[it.job];
+      const StoreView& st = J.st;
+      const int n2 = st.m[it.text], m2 = st.m[it.query], c2 = st.bc[it.query] + 1;
+      if (a.skip_absorbed && (J.acc[it.query] & J.state->tmask & ((1ull << it.slot) - 1ull))) {
+        if (sl == 0) ++nskip;
+        continue;
+      }
+      if (c2 > min(n2, m2)) {
+        if (sl == 0) {
+          if (a.result) a.result[got] = 0;
+          ++nrej;
+        }
+        continue;
+      }
+      item = got;
+      tx = it.text;
+      q = it.query;
+      job = it.job;
+      slot = it.slot;
+      pk = st.pk;
+      n = n2;
+      mq = m2;
+      c = c2;
+      toff = st.pkoff[tx];
+      qoff = st.pkoff[q];
+      ntw = (n >> 5) + 1;
+      b = 0;
+      jb = 0;
+      frozen = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int gw = sl * W + w;
+        uint2 t = make_uint2(0u, 0u);
+        if (gw < ntw) t = pk[toff + gw];
+        tl[w] = t.x;
+        th[w] = t.y;
+        v[w] = ~0u;
       }
     }
     if (__builtin_amdgcn_ballot_w64(!exhausted) == 0) break;
     const bool active = item >= 0;
-    // trivially impossible pairs end at the first block end (LCS <= min(n, m))
-    const bool hopeless = active && (c > min(n, mq));
+    const bool hopeless = false;
 
     // ---- slide the window by one lane when the band has left the lowest lane
     const int lo0 = jb * 32 - (mq - c);
@@ -445,10 +471,12 @@ __device__ inline void align_loop(const AlignArgs& a) {
     nacc += __shfl_xor(nacc, off);
     nrej += __shfl_xor(nrej, off);
     cdec += __shfl_xor(cdec, off);
+    nskip += __shfl_xor(nskip, off);
   }
   if (lane == 0 && a.stats) {
     if (cells) atomicAdd(&a.stats[ST_CELLS_EXEC], cells);
     if (cdec) atomicAdd(&a.stats[ST_CELLS_DECIDED], cdec);
+    if (nskip) atomicAdd(&a.stats[ST_SPEC_WASTE], nskip);
     if (nacc) atomicAdd(&a.stats[ST_ACCEPT], nacc);
     if (nrej) atomicAdd(&a.stats[ST_REJ_ALIGN], nrej);
   }
@@ -465,6 +493,7 @@ __global__ __launch_bounds__(256) void align_kernel(AlignArgs a) {
 hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s);
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
+hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
                             int32_t* jobs_left, hipStream_t s);
 
