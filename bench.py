@@ -43,7 +43,7 @@ def cpu_baseline(cfg_name, threads):
     import oracle_ffi as O
 
     gen, seed, n, t, _ = CONFIGS[cfg_name]
-    sample_n = {"c1": 10000, "c2": 20000, "c4": 100000}[cfg_name]
+    sample_n = {"c1": 10000, "c2": 20000, "c4": 60000}[cfg_name]
     recs = gsdr.Records.generate(gen, seed, sample_n)
     with tempfile.TemporaryDirectory() as d:
         fa = os.path.join(d, "sample.fasta")
@@ -57,17 +57,41 @@ def cpu_baseline(cfg_name, threads):
             "reps": st["reps"]}
 
 
+def copy_calibration():
+    """One 1 GiB device-to-device hipMemcpy: the copy kernel's FETCH_SIZE and
+    WRITE_SIZE per dispatch calibrate the counters against known bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default=os.environ.get("GSDR_BENCH_CONFIG", "c2"), choices=sorted(CONFIGS))
+    ap.add_argument("--config", default=os.environ.get("GSDR_BENCH_CONFIG", "c4"), choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=0, help="override the record count")
     ap.add_argument("--mode", default="batched", choices=["plain", "batched"])
-    ap.add_argument("--compare", action="store_true", help="also time the other form once")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other form")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump", default="", help="write cluster[] (int32, sorted order) to this file")
+    ap.add_argument("--threshold", type=float, default=0.0, help="override the workload's threshold")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -78,6 +102,9 @@ def main():
     gen, seed, n, t, desc = CONFIGS[args.config]
     if args.n:
         n = args.n
+    if args.threshold:
+        t = args.threshold
+        desc = desc.rsplit(", t=", 1)[0] + f", t={t}"
     recs = gsdr.Records.generate(gen, seed, n)
     recs.sort()
     k = gsdr.auto_word_length(t)
@@ -107,6 +134,8 @@ def main():
     # (on the engine's stream) for the kernel-level figures
     clt, st = store.cluster(t, k, mode=mode, timing=True)
     assert (clt == ref).all(), "timed step differs"
+    if args.dump:
+        ref.astype("<i4").tofile(args.dump)
     print(f"# timing step: {st.ms_total:.1f} ms, align {st.ms_align:.1f} ms, filter {st.ms_filter:.1f} ms",
           file=sys.stderr, flush=True)
     bases = recs.bases
@@ -149,7 +178,7 @@ def main():
         gbs = st.filter_bytes / filt_s / 1e9
         out["filter_roofline"] = {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                   "frac": round(gbs / PEAK_HBM_GBS, 5)}
-    if args.compare:
+    if not args.no_compare:
         other = gsdr.MODE_PLAIN if mode == gsdr.MODE_BATCHED else gsdr.MODE_BATCHED
         t0 = time.perf_counter()
         cl2, st2 = store.cluster(t, k, mode=other)
@@ -158,6 +187,9 @@ def main():
         out["other_form"] = {"form": "plain" if other == gsdr.MODE_PLAIN else "batched",
                              "ms": round(dt * 1e3, 3), "mbp_per_s": round(bases / 1e6 / dt, 3),
                              "rounds": int(st2.rounds)}
+        plain_ms = dt * 1e3 if other == gsdr.MODE_PLAIN else ms
+        batched_ms = ms if other == gsdr.MODE_PLAIN else dt * 1e3
+        out["batched_over_plain"] = round(plain_ms / batched_ms, 3)
     store.free()
     eng.close()
     if not args.no_cpu:
