@@ -133,3 +133,15 @@ def test_jobs_alone_equals_in_mix(engine):
     finally:
         for s in stores:
             s.free()
+
+
+@pytest.mark.parametrize("mode", [gsdr.MODE_PLAIN, gsdr.MODE_BATCHED])
+def test_reference_result_subset_fixed_point(engine, tmp_path, mode):
+    """The reference's own output re-clustered at t=0.95 keeps every record
+    (closest pairs sit exactly at LCS == ceil(L*t), rejected by the strict >)."""
+    fa = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "result_subset.fasta")
+    r = gsdr.Records.read_fasta(fa)
+    r.sort()
+    cl, st = gsdr.cluster_records(engine, r, 0.95, mode=mode)
+    assert (cl == np.arange(len(r))).all()
+    assert st.aligned > 0 and st.accepted == 0

@@ -1,0 +1,141 @@
+"""CPU tests of the product's host side and the C ABI boundary (no GPU calls).
+
+- libgsdr.so loads and exports every function include/gsdr.h declares;
+- records: generator determinism and sizes, FASTA round trip, writers;
+- the `cluster` CLI keeps the reference's option handling (func.cpp:2-57);
+- the reference's shipped result.fasta subset is a fixed point of the oracle
+  (idempotence known answer).
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gsdr
+import oracle_ffi as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gsdr.h")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+HAS_GPU = os.path.exists("/dev/kfd")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gsdr_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25
+    out = subprocess.run(["nm", "-D", "--defined-only", gsdr.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (gsdr_[a-z_]+)\b", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # and the Python binding covers them all
+    bound = {name for name, _, _ in gsdr.SIGNATURES}
+    assert set(names) <= bound
+
+
+def test_library_loads_with_signatures():
+    L = gsdr.lib()
+    assert L.gsdr_strerror(0) == b"ok"
+    assert L.gsdr_strerror(gsdr.E_LIMIT) == b"size limit exceeded"
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the no-device error path")
+def test_open_fails_loudly_without_device():
+    with pytest.raises(gsdr.GsdrError) as e:
+        gsdr.Engine(0)
+    assert e.value.status == gsdr.E_DEVICE
+
+
+def test_generator_deterministic_and_sized():
+    a = gsdr.Records.generate(1, 1001)
+    b = gsdr.Records.generate(1, 1001)
+    assert len(a) == 10000 and 8.5e6 < a.bases < 9.5e6
+    assert [a.get(i) for i in range(0, 10000, 997)] == [b.get(i) for i in range(0, 10000, 997)]
+    c = gsdr.Records.generate(2, 1002, 5000)
+    assert len(c) == 5000
+    lens = [len(c.get(i)[1]) for i in range(5000)]
+    assert min(lens) >= 100 and max(lens) <= 7000
+    d = gsdr.Records.generate(4, 1004, 20000)
+    lens = np.array([len(d.get(i)[1]) for i in range(20000)])
+    assert lens.min() >= 90 and lens.max() <= 31000
+    assert abs(np.median(lens) - 160) < 40  # Pareto(1.5, 100): median 100 * 2^(2/3) = 159
+    e = gsdr.Records.generate(1, 1002)
+    assert a.get(0) != e.get(0)
+
+
+def test_fasta_roundtrip_and_sort(tmp_path):
+    r = gsdr.Records.generate(2, 7, 700)
+    p = tmp_path / "x.fasta"
+    r.write_fasta(str(p), 80)
+    for line in p.read_bytes().split(b"\n"):
+        assert len(line) <= 80 or line.startswith(b">")
+    r2 = gsdr.Records.read_fasta(str(p))
+    assert len(r2) == len(r)
+    assert [r2.get(i) for i in range(len(r))] == [r.get(i) for i in range(len(r))]
+    assert O.read_fasta(str(p)) == [r.get(i) for i in range(len(r))]
+    perm = r2.sort()
+    lens = np.array([len(r.get(i)[1]) for i in range(len(r))])
+    assert np.array_equal(perm, O.sort_order(lens))
+
+
+def test_result_and_membership_writers(tmp_path):
+    r = gsdr.Records.from_lists([b"ACGTAC", b"ACGA", b"AC"], names=[b">a", b">b", b">c"])
+    cl = np.array([0, 0, 2], dtype=np.int32)
+    assert r.write_result(cl, str(tmp_path / "r.fasta")) == 2
+    assert (tmp_path / "r.fasta").read_bytes() == b">a\nACGTAC\n>c\nAC\n"
+    r.write_membership(np.array([2, 0, 1]), cl, str(tmp_path / "m.tsv"))
+    assert (tmp_path / "m.tsv").read_text() == "0\t2\t0\t>a\n1\t0\t0\t>b\n2\t1\t2\t>c\n"
+
+
+def run_cli(*args, cwd=None):
+    return subprocess.run([gsdr.CLI_PATH, *args], capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+def test_cli_usage_errors_exit_zero(tmp_path):
+    # odd argument count, unknown option, t out of range, w out of range: usage, exit 0
+    for args in (("t",), ("x", "1"), ("t", "0.7"), ("t", "1.0"), ("w", "3"), ("w", "9"), ("t", "abc")):
+        p = run_cli(*args, cwd=tmp_path)
+        assert p.returncode == 0, args
+        assert "use like this" in p.stdout, args
+    p = run_cli("t", "0.5", cwd=tmp_path)
+    assert "similarity out of range" in p.stdout
+    p = run_cli("w", "12", cwd=tmp_path)
+    assert "word length out of range" in p.stdout
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the no-device error path")
+def test_cli_without_device_fails_with_message(tmp_path):
+    (tmp_path / "in.fasta").write_text(">a\nACGT\n")
+    p = run_cli("-i", "in.fasta", "-t", "0.9", cwd=tmp_path)
+    assert p.returncode == 1
+    assert "input file:\tin.fasta" in p.stdout and "similarity:\t0.9" in p.stdout
+    assert "word length:\t5" in p.stdout
+    assert "HIP" in p.stderr or "device" in p.stderr
+
+
+def test_reference_result_subset_is_fixed_point(tmp_path):
+    fa = os.path.join(GOLDEN, "result_subset.fasta")
+    recs = O.read_fasta(fa)
+    assert len(recs) == 200
+    cl, perm, st = O.cluster_file(fa, 0.95, out_path=str(tmp_path / "o.fasta"))
+    assert (cl == np.arange(len(recs))).all()
+    assert st["aligned"] > 0 and st["accepted"] == 0
+    # output = the same records, in length order
+    names = sorted(n for n, _ in recs)
+    out_names = sorted(l for l in (tmp_path / "o.fasta").read_bytes().split(b"\n") if l.startswith(b">"))
+    assert out_names == names
+
+
+def test_reference_result_subset_full_lcs(tmp_path):
+    # the same known answer with the oracle's full (unbanded) LCS
+    fa = os.path.join(GOLDEN, "result_subset.fasta")
+    cl, _, _ = O.cluster_file(fa, 0.95, fast=False)
+    assert (cl == np.arange(len(cl))).all()
