@@ -171,6 +171,7 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return fail(GSDR_E_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  HIP_TRY(hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * 4, hipHostMallocDefault), "hipHostMalloc");
   c->h_state_cap = 4;
   for (int i = 0; i < kNumShapes; ++i) {
@@ -192,6 +193,7 @@ void gsdr_close(gsdr_ctx* c) {
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->upload) (void)hipStreamDestroy(c->upload);
   if (c->h_state) (void)hipHostFree(c->h_state);
   delete c;
 }

@@ -19,21 +19,6 @@ namespace gsdr {
     if (_e != hipSuccess) return hip_fail(_e, what); \
   } while (0)
 
-namespace {
-
-struct StepLaunch {
-  FilterArgs fa_reps, fa_main;
-  ResolveArgs ra;
-  AlignArgs aa_reps[kNumShapes];
-  AlignArgs aa[kNumShapes];
-  int smax = 0;
-  int max_k = 4;
-  int filter_grid = 1;
-  int align_grid[kNumShapes] = {};
-  int reps_grid[kNumShapes] = {};
-  int reps_filter_grid = 1;
-};
-
 // One step for every job. Phase A: the speculative representatives against
 // each other, then resolve_reps fixes the true ones. Phase B: filter,
 // alignment (skipping candidates an earlier true representative already
@@ -55,7 +40,50 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   return launch_resolve(L.ra, q);
 }
 
-}  // namespace
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left) {
+  Workspace& w = c->ws;
+  StepLaunch L;
+  L.max_k = max_k;
+  L.smax = smax;
+  L.filter_grid = c->cu_count * 8;
+  L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    L.align_grid[sh] = c->cu_count * 4;
+    // phase A holds at most k_spec^2/2 pairs per job
+    const int segs = 64 / kShapes[sh].s;
+    L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));
+  }
+  L.fa_main.jobs = w.jobs;
+  L.fa_main.njobs = njobs;
+  L.fa_main.min_shape = 0;
+  L.fa_main.phase = PHASE_MAIN;
+  L.fa_main.items = w.items;
+  L.fa_main.item_cap = w.item_cap;
+  L.fa_main.bin_count = w.counters;
+  L.fa_main.stats = w.stats;
+  L.fa_reps = L.fa_main;
+  L.fa_reps.phase = PHASE_REPS;
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    AlignArgs& aa = L.aa[sh];
+    aa.jobs = w.jobs;
+    aa.items = w.items + (int64_t)sh * w.item_cap;
+    aa.count = w.counters + sh;
+    aa.work = w.counters + kNumShapes + sh;
+    aa.result = nullptr;
+    aa.stats = w.stats;
+    aa.skip_absorbed = 1;
+    L.aa_reps[sh] = aa;
+    L.aa_reps[sh].skip_absorbed = 0;
+  }
+  L.ra.jobs = w.jobs;
+  L.ra.njobs = njobs;
+  L.ra.counters = w.counters;
+  L.ra.ncounters = 2 * kNumShapes;
+  L.ra.jobs_left = jobs_left;
+  L.ra.stats = w.stats;
+  L.ra.pair_budget = w.item_cap;
+  return L;
+}
 
 int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
                     int32_t* const* out, gsdr_stats* stats) {
@@ -92,6 +120,8 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   // representatives over every job's candidates
   const int64_t item_cap = std::min<int64_t>(std::max<int64_t>(8 * total_n, max_n), 32ll << 20);
   if ((st = workspace_reserve(c, item_cap, njobs))) return st;
+  // the first step queues every job's whole candidate list once
+  if (total_n > c->ws.item_cap) return fail(GSDR_E_LIMIT, "too many sequences in one call");
   Workspace& w = c->ws;
   hipStream_t q = c->stream;
   const auto t0 = std::chrono::steady_clock::now();
@@ -103,48 +133,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs.data(), njobs, k_max, w.jobs_left, q), "init");
 
-  StepLaunch L;
-  L.max_k = max_k;
-  L.smax = smax;
-  L.filter_grid = c->cu_count * 8;
-  L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
-  for (int sh = 0; sh < kNumShapes; ++sh) {
-    L.align_grid[sh] = c->cu_count * 4;
-    // phase A holds at most k_spec^2/2 pairs per job
-    const int segs = 64 / kShapes[sh].s;
-    L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));
-  }
-  L.fa_main.jobs = w.jobs;
-  L.fa_main.njobs = njobs;
-  L.fa_main.min_shape = 0;
-  L.fa_main.phase = PHASE_MAIN;
-  L.fa_main.items = w.items;
-  L.fa_main.item_cap = w.item_cap;
-  L.fa_main.bin_count = w.counters;
-  L.fa_main.stats = w.stats;
-  L.fa_reps = L.fa_main;
-  L.fa_reps.phase = PHASE_REPS;
-  for (int sh = 0; sh < kNumShapes; ++sh) {
-    AlignArgs& aa = L.aa[sh];
-    aa.jobs = w.jobs;
-    aa.items = w.items + (int64_t)sh * w.item_cap;
-    aa.count = w.counters + sh;
-    aa.work = w.counters + kNumShapes + sh;
-    aa.result = nullptr;
-    aa.stats = w.stats;
-    aa.skip_absorbed = 1;
-  }
-  for (int sh = 0; sh < kNumShapes; ++sh) {
-    L.aa_reps[sh] = L.aa[sh];
-    L.aa_reps[sh].skip_absorbed = 0;
-  }
-  L.ra.jobs = w.jobs;
-  L.ra.njobs = njobs;
-  L.ra.counters = w.counters;
-  L.ra.ncounters = 2 * kNumShapes;
-  L.ra.jobs_left = w.jobs_left;
-  L.ra.stats = w.stats;
-  L.ra.pair_budget = w.item_cap;
+  const StepLaunch L = make_step_launch(c, njobs, smax, max_k, w.jobs_left);
 
   double ms_filter = 0, ms_align = 0;
   int64_t launches = 2, replays = 0, steps_launched = 0;
@@ -215,6 +204,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   for (int j = 0; j < njobs; ++j)
     HIP_TRY(hipMemcpyAsync(&hs[j], jobs[j].state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");
+  if (hst[ST_OVERFLOW]) return fail(GSDR_E_LIMIT, "alignment list overflow");
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

@@ -49,7 +49,7 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.state, (size_t)1)))
     return st;
-  hipStream_t q = c->stream;
+  hipStream_t q = c->upload;
   HIP_TRY(hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload pk");
   HIP_TRY(hipMemcpyAsync(s.pkoff, p.pk_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, q), "upload pkoff");
   HIP_TRY(hipMemcpyAsync(s.m_d, p.m.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, q), "upload m");
@@ -112,7 +112,7 @@ int store_set_threshold(gsdr_store& s, float t) {
     bc[i] = base_cutoff(s.raw_len[i], t);
     wc[i] = word_cutoff(s.raw_len[i], t, s.k);
   }
-  hipStream_t q = s.ctx->stream;
+  hipStream_t q = s.ctx->upload;
   HIP_TRY(hipMemcpyAsync(s.bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice, q), "upload bc");
   HIP_TRY(hipMemcpyAsync(s.wc, wc.data(), wc.size() * 4, hipMemcpyHostToDevice, q), "upload wc");
   HIP_TRY(hipStreamSynchronize(q), "cutoff sync");
@@ -275,6 +275,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   unsigned long long hst[ST_NUM];
   HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");
+  if (hst[ST_OVERFLOW]) return fail(GSDR_E_LIMIT, "alignment list overflow");
   if (p.timing)
     for (auto& e : ev) (void)hipEventDestroy(e);
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

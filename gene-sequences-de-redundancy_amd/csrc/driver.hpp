@@ -30,7 +30,8 @@ struct Workspace {
 
 struct gsdr_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // kernels
+  hipStream_t upload = nullptr;       // store uploads (may overlap kernels of another job)
   int cu_count = 0;
   int64_t hbm_bytes = 0;
   char name[256] = {0};
@@ -115,4 +116,24 @@ int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_p
 int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq, const int32_t* pairs,
                 const int32_t* bc, int npairs, int shape, int32_t* accept_out, double* cells_exec_out);
 void stats_from_device(const unsigned long long* d, gsdr_stats* out);
+}  // namespace gsdr
+
+#include "kernels_impl.hpp"
+
+namespace gsdr {
+// Kernel arguments of one two-phase step over the context's job table.
+struct StepLaunch {
+  FilterArgs fa_reps, fa_main;
+  ResolveArgs ra;
+  AlignArgs aa_reps[kNumShapes];
+  AlignArgs aa[kNumShapes];
+  int smax = 0;
+  int max_k = 4;
+  int filter_grid = 1;
+  int align_grid[kNumShapes] = {};
+  int reps_grid[kNumShapes] = {};
+  int reps_filter_grid = 1;
+};
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left);
+hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev);
 }  // namespace gsdr

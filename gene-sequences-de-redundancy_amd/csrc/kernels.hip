@@ -148,6 +148,31 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
   }
 }
 
+// Bounds k_spec of every running job so that k_spec * nc summed over the
+// njobs jobs fits the alignment lists (needed when jobs join a table whose
+// k_spec values were chosen for fewer jobs). Any k_spec is exact.
+__global__ void clamp_spec_kernel(const JobDev* __restrict__ jobs, int njobs, long long pair_budget) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  JobState* S = jobs[j].state;
+  if (S->done) return;
+  const long long cap = pair_budget / ((long long)njobs * (long long)max(S->nc, 1));
+  int k = S->k_spec;
+  if ((long long)k > cap) k = (int)max(cap, 1ll);
+  if (k > S->nc) k = S->nc;
+  if (k < 1) k = 1;
+  if (k != S->k_spec) {
+    S->k_spec = k;
+    S->tmask = 1ull;
+    S->kt = 1;
+  }
+}
+
+hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s) {
+  hipLaunchKernelGGL(clamp_spec_kernel, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs, njobs, pair_budget);
+  return hipGetLastError();
+}
+
 // Initial state of every job: all clusters -1, candidates 0..n-1, one
 // representative slot.
 __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict__ jobs, int k_max) {

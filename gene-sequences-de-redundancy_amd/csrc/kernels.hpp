@@ -62,7 +62,7 @@ __host__ __device__ inline double band_cells_rows(int n, int m, int c, int R) {
 enum StatSlot {
   ST_PAIRS = 0, ST_REJ_LEN, ST_REJ_COMP, ST_REJ_WORD, ST_ALIGNED, ST_ACCEPT, ST_REJ_ALIGN,
   ST_CELLS_EXEC, ST_CELLS_BAND, ST_CELLS_FULL, ST_FILTER_BYTES, ST_SPEC_WASTE, ST_CELLS_DECIDED,
-  ST_NUM = 16
+  ST_OVERFLOW, ST_NUM = 16
 };
 
 // Store arrays resident in HBM (indexed by sorted sequence index).

@@ -95,7 +95,7 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
 //     min(c_r(w), c_q(w)) < wc_q rejects, with c_r staged in LDS.
 // Survivors are appended to the alignment list of their tile shape.
 __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
-  extern __shared__ uint32_t lds[];
+  HIP_DYNAMIC_SHARED(uint32_t, lds)
   uint16_t* table = reinterpret_cast<uint16_t*>(lds);
   __shared__ int s_pre[kMaxJobsPerLaunch + 1];
   __shared__ int s_part[4];
@@ -215,7 +215,14 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
         it.query = q;
         it.job = job;
         it.slot = slot;
-        a.items[(int64_t)sh * a.item_cap + at] = it;
+        if (at < a.item_cap) {
+          a.items[(int64_t)sh * a.item_cap + at] = it;
+        } else {
+          // the list is full: count it so the host fails the call (the host
+          // bounds k_spec * nc so that this does not happen)
+          atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
+          atomicSub(&a.bin_count[sh], 1);
+        }
       }
     }
   }
@@ -494,6 +501,7 @@ hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
+hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
                             int32_t* jobs_left, hipStream_t s);
 

@@ -1,0 +1,132 @@
+"""Continuous-batching server on top of gsdr_serve_* (include/gsdr.h).
+
+Server wraps one device server; run_stream replays a seeded Poisson arrival
+stream of clustering jobs: a loader thread submits each job at its arrival
+time (host preprocessing and upload overlap the device steps), jobs that do
+not fit the HBM budget or a slot wait and retry, and the main thread keeps
+stepping every admitted job in shared launches until all results are in.
+"""
+import ctypes as C
+import threading
+import time
+
+import numpy as np
+
+from . import (E_BUSY, OK, Params, Records, Stats, _check, _i32p, auto_word_length, lib)
+
+
+class Server:
+    def __init__(self, engine, hbm_budget, max_slots):
+        self.h = C.c_void_p()
+        _check(lib().gsdr_serve_open(engine.h, int(hbm_budget), int(max_slots), C.byref(self.h)), "serve_open")
+
+    def submit(self, sorted_records, threshold, word_length=0, spec_reps=0):
+        """Returns a job id, or None when the job must wait (no slot or budget)."""
+        p = Params()
+        p.threshold = threshold
+        p.word_length = word_length or auto_word_length(threshold)
+        p.spec_reps = spec_reps
+        jid = C.c_int64()
+        st = lib().gsdr_serve_submit(self.h, sorted_records.h, C.byref(p), C.byref(jid))
+        if st == E_BUSY:
+            return None
+        _check(st, "serve_submit")
+        return jid.value
+
+    def step(self):
+        running = C.c_int32()
+        _check(lib().gsdr_serve_step(self.h, C.byref(running)), "serve_step")
+        return running.value
+
+    def poll(self, job_id, n):
+        out = np.zeros(n, dtype=np.int32)
+        st = Stats()
+        done = C.c_int32()
+        _check(lib().gsdr_serve_poll(self.h, job_id, _i32p(out), C.byref(st), C.byref(done)), "serve_poll")
+        return (out, st) if done.value else None
+
+    def close(self):
+        if self.h:
+            lib().gsdr_serve_close(self.h)
+            self.h = C.c_void_p()
+
+
+def make_jobs(seed, count, size_range=(1000, 200000), thresholds=(0.80, 0.90, 0.95, 0.99), median=1200.0):
+    """Job specs of the serving workload: log-uniform sizes, thresholds drawn
+    from the set, sequences generated like the config-2 families."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(size_range[0]), np.log(size_range[1])
+    specs = []
+    for j in range(count):
+        n = int(round(np.exp(rng.uniform(lo, hi))))
+        t = float(rng.choice(thresholds))
+        specs.append({"id": j, "n": n, "t": t, "seed": int(seed * 1000003 + j)})
+    return specs
+
+
+def generate_job(spec, median=1200.0):
+    r = Records.generate(2, spec["seed"], spec["n"], median)
+    r.sort()
+    return r
+
+
+def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step_sleep=0.0):
+    """Replays a Poisson stream (rate jobs/s) of pre-generated jobs.
+
+    Returns per-job dicts with latency (arrival to result) and the clusters."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(specs))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = Server(engine, hbm_budget, max_slots)
+    lock = threading.Lock()
+    submitted = {}   # job id -> (spec index, arrival time)
+    results = {}
+    error = []
+    t0 = time.perf_counter()
+
+    def loader():
+        try:
+            for i, spec in enumerate(specs):
+                wait = t0 + arrivals[i] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                while True:
+                    jid = srv.submit(records[i], spec["t"])
+                    if jid is not None:
+                        with lock:
+                            submitted[jid] = (i, t0 + arrivals[i])
+                        break
+                    time.sleep(0.005)
+        except Exception as e:  # surfaced by the stepping thread
+            error.append(e)
+
+    th = threading.Thread(target=loader, daemon=True)
+    th.start()
+    while len(results) < len(specs):
+        if error:
+            raise error[0]
+        running = srv.step()
+        with lock:
+            pending = list(submitted.items())
+        for jid, (i, t_arr) in pending:
+            got = srv.poll(jid, len(records[i]))
+            if got is not None:
+                t_done = time.perf_counter()
+                results[i] = {"job": i, "n": specs[i]["n"], "t": specs[i]["t"], "bases": records[i].bases,
+                              "latency_s": t_done - t_arr, "cluster": got[0], "reps": int(got[1].reps),
+                              "steps": int(got[1].rounds)}
+                with lock:
+                    del submitted[jid]
+        if running == 0 and step_sleep:
+            time.sleep(step_sleep)
+        elif running == 0:
+            time.sleep(0.001)
+    th.join()
+    wall = time.perf_counter() - t0
+    srv.close()
+    out = [results[i] for i in range(len(specs))]
+    lat = np.array([r["latency_s"] for r in out])
+    summary = {"jobs": len(out), "rate_jobs_per_s": rate, "wall_s": wall,
+               "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+               "sustained_mbp_per_s": sum(r["bases"] for r in out) / 1e6 / wall}
+    return out, summary
