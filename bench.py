@@ -57,6 +57,116 @@ def cpu_baseline(cfg_name, threads):
             "reps": st["reps"]}
 
 
+def parallel_map(fn, items, threads=16):
+    """Runs fn over items on host threads (the generator releases the GIL)."""
+    import concurrent.futures as cf
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(fn, items))
+
+
+def bench_jobs(args):
+    """Config 3: independent jobs (log-uniform sizes, per-job median length
+    and threshold) clustered together in shared batched launches."""
+    import numpy as np
+
+    import gsdr
+
+    rng = np.random.default_rng(1003)
+    nj = args.jobs_count
+    lo, hi = np.log(500), np.log(50000 * args.jobs_scale)
+    specs = [{"n": int(round(np.exp(rng.uniform(lo, hi)))), "median": float(rng.choice([150, 400, 1500, 5000])),
+              "t": float(rng.choice([0.80, 0.85, 0.90, 0.95, 0.99])), "seed": int(1003 * 1000003 + j)}
+             for j in range(nj)]
+
+    def gen(s):
+        r = gsdr.Records.generate(2, s["seed"], s["n"], s["median"])
+        r.sort()
+        return r
+
+    t0 = time.perf_counter()
+    recs = parallel_map(gen, specs)
+    print(f"# generated {nj} jobs, {sum(len(r) for r in recs)} records in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    eng = gsdr.Engine(0)
+    t0 = time.perf_counter()
+    stores = []
+    for s, r in zip(specs, recs):
+        st = eng.load(r, gsdr.auto_word_length(s["t"]))
+        stores.append(st)
+    load_s = time.perf_counter() - t0
+    bases = sum(r.bases for r in recs)
+    print(f"# loaded {bases / 1e6:.1f} Mbp in {load_s:.1f} s, HBM {sum(s.footprint for s in stores) / 2**30:.2f} GiB",
+          file=sys.stderr, flush=True)
+    ts = [s["t"] for s in specs]
+    eng.cluster_jobs(stores, ts)  # warmup
+    times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        outs, st = eng.cluster_jobs(stores, ts)
+        times.append(time.perf_counter() - t0)
+        print(f"# step {i}: {times[-1] * 1e3:.1f} ms, steps {st.rounds}, reps {st.reps}", file=sys.stderr, flush=True)
+    # each job alone must give the same clusters (spot check)
+    same = all(np.array_equal(stores[j].cluster(ts[j], stores[j].k, mode=gsdr.MODE_BATCHED)[0], outs[j])
+               for j in range(0, nj, max(1, nj // 8)))
+    t0 = time.perf_counter()
+    for j in range(nj):
+        stores[j].cluster(ts[j], stores[j].k, mode=gsdr.MODE_PLAIN)
+    plain_s = time.perf_counter() - t0
+    ms = 1e3 * sum(times) / len(times)
+    out = {"metric": "clustered Mbp/s, many independent jobs in shared launches", "value": round(bases / 1e6 / (ms / 1e3), 3),
+           "unit": "Mbp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (seeded generator of the stated law)",
+           "config": {"workload": f"config-3 jobs: {nj} jobs, sizes log-uniform [500, {int(50000 * args.jobs_scale)}], "
+                                  "medians {150,400,1500,5000}, t in {.80,.85,.90,.95,.99}",
+                      "records": int(sum(len(r) for r in recs)), "bases": int(bases)},
+           "plain_sequential_s": round(plain_s, 3), "batched_over_plain": round(plain_s / (ms / 1e3), 3),
+           "alone_equals_mix_sample": bool(same), "reps": int(st.reps)}
+    for s in stores:
+        s.free()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Config 5: Poisson arrivals of clustering jobs at several offered loads,
+    admitted by HBM footprint into the continuous-batching server."""
+    import numpy as np
+
+    import gsdr
+    from gsdr import serve
+
+    rates = [float(x) for x in args.rates.split(",")]
+    eng = gsdr.Engine(0)
+    results = []
+    for ri, rate in enumerate(rates):
+        count = max(4, int(round(rate * args.duration)))
+        specs = serve.make_jobs(1005 + ri, count, size_range=(1000, int(200000 * args.jobs_scale)))
+        t0 = time.perf_counter()
+        recs = parallel_map(serve.generate_job, specs)
+        print(f"# rate {rate}: {count} jobs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        out, summary = serve.run_stream(eng, specs, recs, rate, hbm_budget=int(args.budget_gb * 2**30),
+                                        max_slots=args.slots, seed=ri, loaders=3)
+        check = []
+        for r, s, rec in list(zip(out, specs, recs))[:3]:
+            cl, _ = gsdr.cluster_records(eng, rec, s["t"], mode=gsdr.MODE_BATCHED)
+            check.append(bool(np.array_equal(cl, r["cluster"])))
+        summary["alone_equals_mix_first3"] = all(check)
+        summary["offered_mbp_per_s"] = sum(r.bases for r in recs) / 1e6 / max(1e-9, count / rate)
+        print(f"# rate {rate}: {json.dumps(summary)}", file=sys.stderr, flush=True)
+        results.append(summary)
+    eng.close()
+    best = max(results, key=lambda s: s["sustained_mbp_per_s"])
+    print(json.dumps({"metric": "serving: sustained Mbp/s and p50/p99 job latency under Poisson arrivals",
+                      "value": round(best["sustained_mbp_per_s"], 3), "unit": "Mbp/s", "n_gpus": 1,
+                      "higher_is_better": True, "data": "synthetic (seeded generator of the stated law)",
+                      "config": {"workload": f"config-5 stream: sizes log-uniform [1000, {int(200000 * args.jobs_scale)}], "
+                                             f"t in {{.80,.90,.95,.99}}, {args.duration:.0f} s of arrivals per rate",
+                                 "budget_gb": args.budget_gb, "slots": args.slots},
+                      "rates": results}), flush=True)
+
+
 def copy_calibration():
     """One 1 GiB device-to-device hipMemcpy: the copy kernel's FETCH_SIZE and
     WRITE_SIZE per dispatch calibrate the counters against known bytes."""
@@ -88,9 +198,23 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.0, help="override the workload's threshold")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
+    ap.add_argument("--jobs", action="store_true", help="config 3: many jobs in shared launches")
+    ap.add_argument("--jobs-count", type=int, default=512)
+    ap.add_argument("--jobs-scale", type=float, default=1.0, help="scale the upper job size")
+    ap.add_argument("--serve", action="store_true", help="config 5: Poisson serving stream")
+    ap.add_argument("--rates", default="0.5,1,2,4")
+    ap.add_argument("--duration", type=float, default=20.0, help="seconds of arrivals per rate")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--slots", type=int, default=64)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.jobs:
+        bench_jobs(args)
+        return
+    if args.serve:
+        bench_serve(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (every box has one GPU); "

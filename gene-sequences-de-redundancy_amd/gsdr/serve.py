@@ -70,10 +70,12 @@ def generate_job(spec, median=1200.0):
     return r
 
 
-def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step_sleep=0.0):
+def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step_sleep=0.0, loaders=1):
     """Replays a Poisson stream (rate jobs/s) of pre-generated jobs.
 
-    Returns per-job dicts with latency (arrival to result) and the clusters."""
+    `loaders` host threads take the jobs in arrival order and submit each at
+    its arrival time. Returns per-job dicts with latency (arrival to result)
+    and the clusters, and a summary."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(specs))
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -82,16 +84,22 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
     submitted = {}   # job id -> (spec index, arrival time)
     results = {}
     error = []
+    nxt = [0]
     t0 = time.perf_counter()
 
     def loader():
         try:
-            for i, spec in enumerate(specs):
+            while True:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= len(specs):
+                    return
                 wait = t0 + arrivals[i] - time.perf_counter()
                 if wait > 0:
                     time.sleep(wait)
                 while True:
-                    jid = srv.submit(records[i], spec["t"])
+                    jid = srv.submit(records[i], specs[i]["t"])
                     if jid is not None:
                         with lock:
                             submitted[jid] = (i, t0 + arrivals[i])
@@ -100,8 +108,9 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
         except Exception as e:  # surfaced by the stepping thread
             error.append(e)
 
-    th = threading.Thread(target=loader, daemon=True)
-    th.start()
+    threads = [threading.Thread(target=loader, daemon=True) for _ in range(max(1, loaders))]
+    for th in threads:
+        th.start()
     while len(results) < len(specs):
         if error:
             raise error[0]
@@ -121,7 +130,8 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
             time.sleep(step_sleep)
         elif running == 0:
             time.sleep(0.001)
-    th.join()
+    for th in threads:
+        th.join()
     wall = time.perf_counter() - t0
     srv.close()
     out = [results[i] for i in range(len(specs))]
