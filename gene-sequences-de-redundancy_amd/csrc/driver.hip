@@ -333,12 +333,15 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
   std::vector<std::vector<int>> bin_pair(kNumShapes);
   for (int i = 0; i < npairs; ++i) {
     const int t = pairs[2 * i];
-    const int sh = pick_shape(p.m[t], m[nseq + i], bcv[nseq + i] + 1, min_shape);
+    const int cslot = nseq + i;  // the query, carrying the pair's cutoff
+    const bool sw = swap_roles(p.m[t], m[cslot]);
+    const int c = bcv[cslot] + 1;
+    const int sh = sw ? pick_shape(m[cslot], p.m[t], c, min_shape) : pick_shape(p.m[t], m[cslot], c, min_shape);
     AlignItem it;
-    it.text = t;
-    it.query = nseq + i;
+    it.text = sw ? cslot : t;
+    it.query = sw ? t : cslot;
     it.job = 0;
-    it.slot = 0;
+    it.slot = sw ? kSwapFlag : 0;
     bins[sh].push_back(it);
     bin_pair[sh].push_back(i);
   }

@@ -124,10 +124,22 @@ struct JobDev {
   int32_t k;                 // word length of the stored profiles
 };
 
-// Alignment work item: text (representative) and query sorted indices, the
-// job and the representative's slot in this step.
+// Alignment work item: text and query sorted indices, the job and the
+// representative's slot in this step. Normally text = representative and
+// query = candidate. LCS is symmetric, so when the representative is at
+// least kSwapRatio times longer the roles swap (flag kSwapFlag in slot): the
+// bit-vector spans the short candidate and rows run over the long
+// representative, where the exact early accept fires after about 4m rows
+// instead of costing n/32 words per row. The cutoff always belongs to the
+// candidate.
 struct AlignItem {
   int32_t text, query, job, slot;
 };
+constexpr int kSwapFlag = 256;
+constexpr int kSwapRatio = 4;
+
+__host__ __device__ inline bool swap_roles(int n_rep, int m_cand) {
+  return n_rep >= kSwapRatio * m_cand;
+}
 
 }  // namespace gsdr

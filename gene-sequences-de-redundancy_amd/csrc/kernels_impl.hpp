@@ -208,13 +208,14 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
         bytes += 16;
         cband += (unsigned long long)band_cells(n, mq, c);
         cfull += (unsigned long long)n * (unsigned long long)mq;
-        const int sh = pick_shape(n, mq, c, a.min_shape);
+        const bool sw = swap_roles(n, mq);
+        const int sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
         const int at = atomicAdd(&a.bin_count[sh], 1);
         AlignItem it;
-        it.text = r;
-        it.query = q;
+        it.text = sw ? q : r;
+        it.query = sw ? r : q;
         it.job = job;
-        it.slot = slot;
+        it.slot = slot | (sw ? kSwapFlag : 0);
         if (at < a.item_cap) {
           a.items[(int64_t)sh * a.item_cap + at] = it;
         } else {
@@ -294,7 +295,9 @@ __device__ inline void align_loop(const AlignArgs& a) {
 
   int item = -1;
   bool exhausted = false;
-  int tx = 0, q = 0, job = 0, slot = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
+  // tx: bit-vector sequence, qx: row sequence, q: the candidate (owner of
+  // the cutoff and of the accept mark)
+  int tx = 0, qx = 0, q = 0, job = 0, slot = 0, n = 0, mq = 0, c = 0, b = 0, jb = 0, frozen = 0, ntw = 0;
   const uint2* pk = nullptr;
   int64_t toff = 0, qoff = 0;
   unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
@@ -319,8 +322,10 @@ __device__ inline void align_loop(const AlignArgs& a) {
       const AlignItem it = a.items[got];
       const JobDev& J = a.jobs[it.job];
       const StoreView& st = J.st;
-      const int n2 = st.m[it.text], m2 = st.m[it.query], c2 = st.bc[it.query] + 1;
-      if (a.skip_absorbed && (J.acc[it.query] & J.state->tmask & ((1ull << it.slot) - 1ull))) {
+      const int islot = it.slot & (kSwapFlag - 1);
+      const int cand = (it.slot & kSwapFlag) ? it.text : it.query;  // the candidate owns the cutoff
+      const int n2 = st.m[it.text], m2 = st.m[it.query], c2 = st.bc[cand] + 1;
+      if (a.skip_absorbed && (J.acc[cand] & J.state->tmask & ((1ull << islot) - 1ull))) {
         if (sl == 0) ++nskip;
         continue;
       }
@@ -333,15 +338,16 @@ __device__ inline void align_loop(const AlignArgs& a) {
       }
       item = got;
       tx = it.text;
-      q = it.query;
+      q = cand;
+      qx = it.query;
       job = it.job;
-      slot = it.slot;
+      slot = islot;
       pk = st.pk;
       n = n2;
       mq = m2;
       c = c2;
       toff = st.pkoff[tx];
-      qoff = st.pkoff[q];
+      qoff = st.pkoff[qx];
       ntw = (n >> 5) + 1;
       b = 0;
       jb = 0;
