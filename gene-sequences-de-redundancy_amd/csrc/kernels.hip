@@ -228,9 +228,13 @@ hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s) 
     case 2: return launch_align_sw<32, 1>(a, grid, s);
     case 3: return launch_align_sw<64, 1>(a, grid, s);
     case 4: return launch_align_sw<64, 2>(a, grid, s);
-    case 5: return launch_align_sw<64, 4>(a, grid, s);
-    case 6: return launch_align_sw<64, 8>(a, grid, s);
-    case 7: return launch_align_sw<64, 16>(a, grid, s);
+    case 5: return launch_align_sw<64, 3>(a, grid, s);
+    case 6: return launch_align_sw<64, 4>(a, grid, s);
+    case 7: return launch_align_sw<64, 6>(a, grid, s);
+    case 8: return launch_align_sw<64, 8>(a, grid, s);
+    case 9: return launch_align_sw<64, 10>(a, grid, s);
+    case 10: return launch_align_sw<64, 12>(a, grid, s);
+    case 11: return launch_align_sw<64, 16>(a, grid, s);
     default: return hipErrorInvalidValue;
   }
 }

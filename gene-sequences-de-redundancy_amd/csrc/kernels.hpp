@@ -8,11 +8,13 @@ namespace gsdr {
 // Alignment tile shapes: S lanes of one wave per pair, W 32-bit text words
 // per lane. A pair fits a shape when the S*W-word window covers the whole
 // text, or when the sliding window ((S-1)*W words plus one lane of lag)
-// covers the diagonal band of the pair.
-constexpr int kNumShapes = 8;
+// covers the diagonal band of the pair. Widths between the powers of two
+// keep the executed window close to the band: a row costs about W word
+// updates per lane whatever part of the window the band uses.
+constexpr int kNumShapes = 12;
 struct Shape { int s, w; };
-constexpr Shape kShapes[kNumShapes] = {{8, 1}, {16, 1}, {32, 1}, {64, 1},
-                                       {64, 2}, {64, 4}, {64, 8}, {64, 16}};
+constexpr Shape kShapes[kNumShapes] = {{8, 1},  {16, 1}, {32, 1}, {64, 1},  {64, 2},  {64, 3},
+                                       {64, 4}, {64, 6}, {64, 8}, {64, 10}, {64, 12}, {64, 16}};
 
 // Smallest shape that holds a pair with text length n, query length m and
 // minimum accepted LCS c (both gap-free lengths). E = n + m - 2c bounds the

@@ -88,7 +88,7 @@ def test_boundary_decisions(engine, seed, count, max_len):
     assert cells > 0
 
 
-@pytest.mark.parametrize("shape", list(range(8)))
+@pytest.mark.parametrize("shape", list(range(12)))
 def test_every_tile_shape(engine, shape):
     """Forces each tile shape (the smallest fitting shape at or above it)."""
     seqs, pairs = make_pairs(100 + shape, 60, 2500)
