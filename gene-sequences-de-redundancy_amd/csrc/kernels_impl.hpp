@@ -264,6 +264,60 @@ __device__ inline uint32_t valid_bits(int gw, int n) {
   return 0u;
 }
 
+// First half of a row for a wave-uniform query base B (2-bit code): match
+// mask, U = V & M and the per-lane multi-word sum S = V + U with its carry
+// out. The code is fixed at compile time, so M & V is one three-input bit
+// operation on (V, lo plane, hi plane).
+template <int B, int W>
+__device__ __forceinline__ void row_sum_base(const uint32_t (&v)[W], const uint32_t (&tl)[W],
+                                             const uint32_t (&th)[W], uint32_t (&s)[W], uint32_t (&u)[W],
+                                             uint32_t& carry) {
+  carry = 0u;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint32_t ml = (B & 1) ? tl[w] : ~tl[w];
+    const uint32_t mh = (B & 2) ? th[w] : ~th[w];
+    u[w] = v[w] & ml & mh;
+    uint32_t co;
+    s[w] = __builtin_addc(v[w], u[w], carry, &co);
+    carry = co;
+  }
+}
+
+// Second half of a row for one 64-lane segment: carry look-ahead across the
+// lanes, then V' = (S + carry in) | (V ^ U). A carry in only ripples past a
+// lane's first word when that word of S is all ones, which is rare, so the
+// ripple through the other words runs only when some lane needs it; the
+// propagate test likewise looks at the other words only when some first word
+// is all ones.
+template <int W>
+__device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], const uint32_t (&u)[W],
+                                             uint32_t carry, int lane) {
+  constexpr uint64_t KTOP = 1ull << 63;
+  const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
+  uint64_t P = 0;
+  if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) != 0) {
+    uint32_t pall = ~0u;
+#pragma unroll
+    for (int w = 0; w < W; ++w) pall &= s[w];
+    P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+  }
+  const uint64_t X = G | P;
+  const uint64_t C = (X + G) ^ X ^ G;
+  uint32_t co;
+  s[0] = __builtin_addc(s[0], 0u, (uint32_t)(C >> lane) & 1u, &co);
+  if (W > 1 && __builtin_amdgcn_ballot_w64(co != 0u) != 0) {
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+      uint32_t c2;
+      s[w] = __builtin_addc(s[w], 0u, co, &c2);
+      co = c2;
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) v[w] = s[w] | (v[w] ^ u[w]);
+}
+
 // Banded bit-parallel LCS decision, LCS(text, query) > bc_query.
 //
 // Recurrence (func.cpp:596-626): V starts all ones over the text; per query
@@ -408,6 +462,28 @@ __device__ inline void align_loop(const AlignArgs& a) {
       qnv = rows >= 32 ? 0u : ~((1u << rows) - 1u);
     }
 
+    if constexpr (S == 64 && W >= 2) {
+      // one pair per wave: the query base of a row is wave-uniform, so each
+      // row branches (scalar) to the code specialised for its base and rows
+      // past the query end are skipped
+      const uint32_t qls = __builtin_amdgcn_readfirstlane(qlo);
+      const uint32_t qhs = __builtin_amdgcn_readfirstlane(qhi);
+      const int nrows = __builtin_amdgcn_readfirstlane(active ? min(32, mq - jb * 32) : 0);
+#pragma unroll 1
+      for (int r = 0; r < nrows; ++r) {
+        // keeps the four match masks from being hoisted out of the row loop
+        // (they would take 4*W more registers)
+#pragma unroll
+        for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
+        const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
+        uint32_t s[W], u[W], carry;
+        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
+        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
+        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
+        else row_sum_base<3, W>(v, tl, th, s, u, carry);
+        row_finish64<W>(v, s, u, carry, lane);
+      }
+    } else {
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
     for (int r0 = 0; r0 < 32; r0 += UR) {
@@ -441,6 +517,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
           v[w] = t | (v[w] & x[w]);
         }
       }
+    }
     }
 
     // ---- block end: exact early decisions on the zero count

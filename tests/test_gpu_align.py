@@ -97,6 +97,32 @@ def test_every_tile_shape(engine, shape):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("shape", [4, 5, 7, 9, 11])
+def test_single_base_runs(engine, shape):
+    """Texts made of long single-base runs: whole words match nothing for most
+    query bases, so row sums are all ones and carries ripple through every word
+    of a lane and across lanes (the rarely taken ripple paths of the 64-lane
+    tiles)."""
+    rng = np.random.default_rng(40 + shape)
+    seqs, pairs = [], []
+    for i in range(10):
+        runs, n = [], int(rng.integers(1500, 6000))
+        while sum(len(r) for r in runs) < n:
+            runs.append(bytes([int(rng.choice(ACGT))]) * int(rng.integers(33, 260)))
+        text = b"".join(runs)
+        if i % 3 == 0:
+            q = rand_seq(rng, int(rng.integers(200, len(text) // 2)))
+        elif i % 3 == 1:
+            q = mutate(rng, text, 0.05)
+        else:
+            q = text[: len(text) // 3] + rand_seq(rng, 400) + text[len(text) // 2:]
+        seqs += [text, q]
+        pairs.append((2 * i, 2 * i + 1))
+    ep, bcs, expect = boundary_cases(seqs, pairs)
+    got, _ = engine.align_pairs(seqs, ep, bcs, shape=shape)
+    assert np.array_equal(got, expect)
+
+
 def test_near_threshold_long(engine):
     """Long similar pairs at t=0.95/0.90 cutoffs and the 32,767-base maximum."""
     rng = np.random.default_rng(7)
