@@ -282,30 +282,55 @@ __device__ __forceinline__ void row_sum_base(const uint32_t (&v)[W], const uint3
     s[w] = __builtin_addc(v[w], u[w], carry, &co);
     carry = co;
   }
+  // the carry leaves the branch as a 0/1 register value: merging a lane-mask
+  // value over the four branches would cost scalar mask arithmetic per row
+  asm volatile("" : "+v"(carry));
+}
+
+// The same first half with the base applied as two wave-uniform masks
+// (ql, qh all ones where the base bit is set): no branch, two more bit
+// operations per word. Cheaper than branching when a lane holds few words.
+template <int W>
+__device__ __forceinline__ void row_sum_masks(const uint32_t (&v)[W], const uint32_t (&tl)[W],
+                                              const uint32_t (&th)[W], uint32_t ql, uint32_t qh,
+                                              uint32_t (&s)[W], uint32_t (&u)[W], uint32_t& carry) {
+  carry = 0u;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    u[w] = v[w] & ~(tl[w] ^ ql) & ~(th[w] ^ qh);
+    uint32_t co;
+    s[w] = __builtin_addc(v[w], u[w], carry, &co);
+    carry = co;
+  }
 }
 
 // Second half of a row for one 64-lane segment: carry look-ahead across the
-// lanes, then V' = (S + carry in) | (V ^ U). A carry in only ripples past a
-// lane's first word when that word of S is all ones, which is rare, so the
-// ripple through the other words runs only when some lane needs it; the
-// propagate test likewise looks at the other words only when some first word
-// is all ones.
+// lanes, then V' = (S + carry in) | (V ^ U). A lane passes a carry in on only
+// when all of its words of S are ones, which needs its first word to be all
+// ones. When no lane has that, the carry into lane l is simply the carry out
+// of lane l-1 (one lane shift of the wave, DPP wave_shr:1); otherwise the full
+// generate/propagate look-ahead runs. Likewise the carry in ripples past a
+// lane's first word only when that word is all ones, so the ripple through
+// the other words runs only when some lane needs it.
 template <int W>
 __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], const uint32_t (&u)[W],
                                              uint32_t carry, int lane) {
   constexpr uint64_t KTOP = 1ull << 63;
-  const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
-  uint64_t P = 0;
-  if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) != 0) {
+  uint32_t cin;
+  if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) == 0) {
+    cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  } else {
+    const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
     uint32_t pall = ~0u;
 #pragma unroll
     for (int w = 0; w < W; ++w) pall &= s[w];
-    P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+    const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+    const uint64_t X = G | P;
+    const uint64_t C = (X + G) ^ X ^ G;
+    cin = (uint32_t)(C >> lane) & 1u;
   }
-  const uint64_t X = G | P;
-  const uint64_t C = (X + G) ^ X ^ G;
   uint32_t co;
-  s[0] = __builtin_addc(s[0], 0u, (uint32_t)(C >> lane) & 1u, &co);
+  s[0] = __builtin_addc(s[0], 0u, cin, &co);
   if (W > 1 && __builtin_amdgcn_ballot_w64(co != 0u) != 0) {
 #pragma unroll
     for (int w = 1; w < W; ++w) {
@@ -463,24 +488,33 @@ __device__ inline void align_loop(const AlignArgs& a) {
     }
 
     if constexpr (S == 64 && W >= 2) {
-      // one pair per wave: the query base of a row is wave-uniform, so each
-      // row branches (scalar) to the code specialised for its base and rows
-      // past the query end are skipped
+      // one pair per wave: the query base of a row is wave-uniform and rows
+      // past the query end are skipped. With 6 or more words per lane each
+      // row branches (scalar) to the code specialised for its base; with
+      // fewer the base enters as uniform masks (measured faster there: the
+      // branch costs more scalar issue than the two bit operations per word
+      // it saves).
       const uint32_t qls = __builtin_amdgcn_readfirstlane(qlo);
       const uint32_t qhs = __builtin_amdgcn_readfirstlane(qhi);
       const int nrows = __builtin_amdgcn_readfirstlane(active ? min(32, mq - jb * 32) : 0);
 #pragma unroll 1
       for (int r = 0; r < nrows; ++r) {
-        // keeps the four match masks from being hoisted out of the row loop
-        // (they would take 4*W more registers)
-#pragma unroll
-        for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
-        const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
         uint32_t s[W], u[W], carry;
-        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
-        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
-        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
-        else row_sum_base<3, W>(v, tl, th, s, u, carry);
+        if constexpr (W >= 6) {
+          // keeps the four match masks from being hoisted out of the row
+          // loop (they would take 4*W more registers)
+#pragma unroll
+          for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
+          const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
+          if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
+          else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
+          else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
+          else row_sum_base<3, W>(v, tl, th, s, u, carry);
+        } else {
+          const uint32_t ql = ((qls >> r) & 1u) ? ~0u : 0u;
+          const uint32_t qh = ((qhs >> r) & 1u) ? ~0u : 0u;
+          row_sum_masks<W>(v, tl, th, ql, qh, s, u, carry);
+        }
         row_finish64<W>(v, s, u, carry, lane);
       }
     } else {
