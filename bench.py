@@ -27,11 +27,11 @@ sys.path.insert(0, PKG)
 PEAK_HBM_GBS = 8000.0
 PEAK_VALU_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
 
-# workloads: (generator config, seed, record count, threshold)
+# workloads: (generator law, seed, record count, threshold, description)
 CONFIGS = {
-    "c1": (1, 1001, 10000, 0.95, "config-1 mutated families: 10,000 records from 500 parents, t=0.95"),
-    "c2": (2, 1002, 200000, 0.90, "config-2 mutated families: 200,000 records, log-normal median 1,200, t=0.90"),
-    "c4": (4, 1004, 2000000, 0.95, "config-4 skewed single job: 2,000,000 records, Pareto(1.5) lengths, t=0.95"),
+    "c1": (1, 1001, 10000, 0.95, "mutated families: 10,000 records from 500 parents, t=0.95"),
+    "c2": (2, 1002, 200000, 0.90, "mutated families: 200,000 records, log-normal median 1,200, t=0.90"),
+    "c4": (4, 1004, 2000000, 0.95, "skewed single job: 2,000,000 records, Pareto(1.5) lengths, t=0.95"),
 }
 
 
@@ -57,6 +57,25 @@ def cpu_baseline(cfg_name, threads):
             "reps": st["reps"]}
 
 
+def committed_traffic(args, nrec, t):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    of this same default workload (scaled by a plain copy of known size),
+    read from the committed summary; None for any other workload."""
+    default = CONFIGS["c4"]
+    if args.config != "c4" or nrec != default[2] or abs(t - default[3]) > 1e-9:
+        return {}
+    path = os.path.join(ROOT, "profiles", "align_filter_c4_t095_pmc.md")
+    out = {}
+    try:
+        for line in open(path):
+            for key, name in (("align", "traffic_per_align_launch_bytes"), ("filter", "traffic_per_filter_launch_bytes")):
+                if line.startswith(name + " = "):
+                    out[key] = int(line.split("=")[1])
+    except OSError:
+        return {}
+    return out
+
+
 def parallel_map(fn, items, threads=16):
     """Runs fn over items on host threads (the generator releases the GIL)."""
     import concurrent.futures as cf
@@ -66,8 +85,8 @@ def parallel_map(fn, items, threads=16):
 
 
 def bench_jobs(args):
-    """Config 3: independent jobs (log-uniform sizes, per-job median length
-    and threshold) clustered together in shared batched launches."""
+    """Many independent jobs (log-uniform sizes, per-job median length and
+    threshold) clustered together in shared batched launches."""
     import numpy as np
 
     import gsdr
@@ -118,7 +137,7 @@ def bench_jobs(args):
            "unit": "Mbp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
            "data": "synthetic (seeded generator of the stated law)",
-           "config": {"workload": f"config-3 jobs: {nj} jobs, sizes log-uniform [500, {int(50000 * args.jobs_scale)}], "
+           "config": {"workload": f"many jobs: {nj} jobs, sizes log-uniform [500, {int(50000 * args.jobs_scale)}], "
                                   "medians {150,400,1500,5000}, t in {.80,.85,.90,.95,.99}",
                       "records": int(sum(len(r) for r in recs)), "bases": int(bases)},
            "plain_sequential_s": round(plain_s, 3), "batched_over_plain": round(plain_s / (ms / 1e3), 3),
@@ -130,7 +149,7 @@ def bench_jobs(args):
 
 
 def bench_serve(args):
-    """Config 5: Poisson arrivals of clustering jobs at several offered loads,
+    """Poisson arrivals of clustering jobs at several offered loads,
     admitted by HBM footprint into the continuous-batching server."""
     import numpy as np
 
@@ -161,7 +180,7 @@ def bench_serve(args):
     print(json.dumps({"metric": "serving: sustained Mbp/s and p50/p99 job latency under Poisson arrivals",
                       "value": round(best["sustained_mbp_per_s"], 3), "unit": "Mbp/s", "n_gpus": 1,
                       "higher_is_better": True, "data": "synthetic (seeded generator of the stated law)",
-                      "config": {"workload": f"config-5 stream: sizes log-uniform [1000, {int(200000 * args.jobs_scale)}], "
+                      "config": {"workload": f"Poisson job stream: sizes log-uniform [1000, {int(200000 * args.jobs_scale)}], "
                                              f"t in {{.80,.90,.95,.99}}, {args.duration:.0f} s of arrivals per rate",
                                  "budget_gb": args.budget_gb, "slots": args.slots},
                       "rates": results}), flush=True)
@@ -198,10 +217,10 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.0, help="override the workload's threshold")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
-    ap.add_argument("--jobs", action="store_true", help="config 3: many jobs in shared launches")
+    ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
     ap.add_argument("--jobs-count", type=int, default=512)
     ap.add_argument("--jobs-scale", type=float, default=1.0, help="scale the upper job size")
-    ap.add_argument("--serve", action="store_true", help="config 5: Poisson serving stream")
+    ap.add_argument("--serve", action="store_true", help="Poisson stream of jobs into the serving loop")
     ap.add_argument("--rates", default="0.5,1,2,4")
     ap.add_argument("--duration", type=float, default=20.0, help="seconds of arrivals per rate")
     ap.add_argument("--budget-gb", type=float, default=200.0)
@@ -291,17 +310,25 @@ def main():
         "kernel_ms": {"align": round(st.ms_align, 3), "filter": round(st.ms_filter, 3),
                       "align_launches": st.align_launches, "filter_launches": st.filter_launches},
     }
+    # one align launch = the tile-shape kernels of one step phase (launched
+    # together, one filter launch each); HIP events bracket each such launch
+    groups = max(1, st.filter_launches)
+    traffic = committed_traffic(args, len(recs), t) if args.mode == "batched" else {}
     if align_s > 0:
-        ops = st.cells_decided / 8.0  # 4 int32 ops per 32-cell word step
-        ach = ops / align_s / 1e12
-        out["roofline"] = {"bound": "valu", "kernel": "align_kernel (all tile shapes)",
+        # algorithmic work: 4 int32 ops per 32-cell word step over the exact
+        # decision band of every aligned pair (cells_band / 8)
+        ach = st.cells_band / 8.0 / align_s / 1e12
+        ach_dec = st.cells_decided / 8.0 / align_s / 1e12
+        out["roofline"] = {"bound": "valu", "kernel": "align_kernel (all tile shapes of a step phase)",
                            "achieved": round(ach, 4), "peak": round(PEAK_VALU_TOPS, 2), "unit": "TOP/s",
-                           "frac": round(ach / PEAK_VALU_TOPS, 5), "traffic": None,
-                           "per_launch_ms": round(st.ms_align / max(1, st.align_launches), 5)}
+                           "frac": round(ach / PEAK_VALU_TOPS, 5), "traffic": traffic.get("align"),
+                           "per_launch_ms": round(st.ms_align / groups, 5),
+                           "achieved_decided_cells": round(ach_dec, 4)}
     if filt_s > 0:
         gbs = st.filter_bytes / filt_s / 1e9
         out["filter_roofline"] = {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                  "frac": round(gbs / PEAK_HBM_GBS, 5)}
+                                  "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": traffic.get("filter"),
+                                  "per_launch_ms": round(st.ms_filter / groups, 5)}
     if not args.no_compare:
         other = gsdr.MODE_PLAIN if mode == gsdr.MODE_BATCHED else gsdr.MODE_BATCHED
         t0 = time.perf_counter()

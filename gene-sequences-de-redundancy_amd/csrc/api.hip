@@ -87,11 +87,12 @@ int gsdr_records_from_buffers(const char* names, const int64_t* name_off, const 
   return GSDR_OK;
 }
 
-int gsdr_records_generate(int32_t config, uint64_t seed, int64_t n, double median, gsdr_records** out) {
-  if (!out || config < 1 || config > 4) return fail(GSDR_E_INVALID, "config must be 1..4");
+int gsdr_records_generate(int32_t law, uint64_t seed, int64_t n, double median, gsdr_records** out) {
+  if (!out || (law != GSDR_LAW_FAMILIES && law != GSDR_LAW_LOGNORMAL && law != GSDR_LAW_PARETO))
+    return fail(GSDR_E_INVALID, "law must be GSDR_LAW_FAMILIES, GSDR_LAW_LOGNORMAL or GSDR_LAW_PARETO");
   auto r = std::make_unique<gsdr_records>();
   GenSpec spec;
-  spec.config = config;
+  spec.law = law;
   spec.seed = seed;
   spec.n_override = n;
   spec.median = median;

@@ -394,7 +394,7 @@ void generate(const GenSpec& spec, Records& out) {
   Rng g(spec.seed);
   std::vector<std::string> names, seqs;
   std::string parent, child;
-  if (spec.config == 1) {
+  if (spec.law == 1) {
     // 500 parents, L ~ U{300..1500}; each parent plus 19 children
     // (sub 0.02, ins 0.005, del 0.005).
     int64_t total = spec.n_override > 0 ? spec.n_override : 10000;
@@ -413,13 +413,13 @@ void generate(const GenSpec& spec, Records& out) {
     to_records(names, seqs, out);
     return;
   }
-  if (spec.config == 2) {
+  if (spec.law == 2) {
     int64_t total = spec.n_override > 0 ? spec.n_override : 200000;
     double median = spec.median > 0 ? spec.median : 1200.0;
     generate_family_job(spec.seed, total, median, out);
     return;
   }
-  if (spec.config == 4) {
+  if (spec.law == 4) {
     // Pareto(1.5, min 100) lengths clipped at 30000; 30% singletons, 70% in
     // families of Geometric(1/8) size whose members are the (unemitted) root
     // mutated at d ~ U[0, 0.06] (80% substitutions, 20% indels); 5% of all
@@ -466,7 +466,7 @@ void generate(const GenSpec& spec, Records& out) {
     to_records(names, seqs, out);
     return;
   }
-  // config 3 jobs and config 5 requests are generated per job through
+  // many-job workloads and stream requests are generated per job through
   // generate_family_job with their own seeds and medians.
   int64_t total = spec.n_override > 0 ? spec.n_override : 10000;
   generate_family_job(spec.seed, total, spec.median > 0 ? spec.median : 1200.0, out);

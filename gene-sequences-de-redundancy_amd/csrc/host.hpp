@@ -70,16 +70,18 @@ struct Prep {
 // Builds Prep from sorted records using `threads` host threads.
 void preprocess(const Records& sorted, int k, int threads, Prep& p);
 
-// Synthetic generator (seeded, deterministic). config 1..4 follow the
-// mutated-family workloads; n_override > 0 rescales the record count.
+// Synthetic generator (seeded, deterministic). law: 1 = 500 parents with 19
+// mutated children each, 2 = log-normal-length parents with 10 children each,
+// 4 = Pareto(1.5) lengths with singletons and families (values match
+// GSDR_LAW_* in gsdr.h); n_override > 0 rescales the record count.
 struct GenSpec {
-  int config = 1;
+  int law = 1;
   uint64_t seed = 1001;
   int64_t n_override = 0;
-  double median = 0;     // override log-normal median (config 2/3 jobs)
+  double median = 0;     // override log-normal median (law 2 and per-job workloads)
 };
 void generate(const GenSpec& spec, Records& out);
-// One job "as in config 2": n sequences, log-normal lengths with the given
+// One job of law 2: n sequences, log-normal lengths with the given
 // median (sigma 0.5) clipped to [150, 6000], 10 children per parent.
 void generate_family_job(uint64_t seed, int64_t n, double median, Records& out);
 

@@ -156,9 +156,9 @@ class Records:
         return Records(h)
 
     @staticmethod
-    def generate(config, seed, n=0, median=0.0):
+    def generate(law, seed, n=0, median=0.0):
         h = C.c_void_p()
-        _check(lib().gsdr_records_generate(config, seed, n, median, C.byref(h)), "generate")
+        _check(lib().gsdr_records_generate(law, seed, n, median, C.byref(h)), "generate")
         return Records(h)
 
     @staticmethod

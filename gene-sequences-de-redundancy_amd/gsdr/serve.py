@@ -53,7 +53,7 @@ class Server:
 
 def make_jobs(seed, count, size_range=(1000, 200000), thresholds=(0.80, 0.90, 0.95, 0.99), median=1200.0):
     """Job specs of the serving workload: log-uniform sizes, thresholds drawn
-    from the set, sequences generated like the config-2 families."""
+    from the set, sequences generated like the 200,000-record mutated families."""
     rng = np.random.default_rng(seed)
     lo, hi = np.log(size_range[0]), np.log(size_range[1])
     specs = []

@@ -89,9 +89,14 @@ int gsdr_records_read_fasta(const char *path, gsdr_records **out);
 /* Builds records from caller buffers (offsets have count+1 entries). */
 int gsdr_records_from_buffers(const char *names, const int64_t *name_off, const char *data,
                               const int64_t *data_off, int64_t count, gsdr_records **out);
-/* Synthetic mutated-family workloads (config 1..4; seed; n > 0 overrides
- * the record count; median > 0 overrides the length median). */
-int gsdr_records_generate(int32_t config, uint64_t seed, int64_t n, double median,
+/* Synthetic workloads: law GSDR_LAW_*; seed; n > 0 overrides the record
+ * count; median > 0 overrides the length median. */
+enum {
+  GSDR_LAW_FAMILIES = 1,         /* 500 parents, 19 mutated children each, L 300..1500 */
+  GSDR_LAW_LOGNORMAL = 2,        /* log-normal lengths, 10 mutated children per parent */
+  GSDR_LAW_PARETO = 4            /* Pareto(1.5) lengths, singletons and families */
+};
+int gsdr_records_generate(int32_t law, uint64_t seed, int64_t n, double median,
                           gsdr_records **out);
 int gsdr_records_write_fasta(const gsdr_records *r, const char *path, int32_t line_width);
 /* Reorders in place into the reference's length order (func.cpp:105-107);
