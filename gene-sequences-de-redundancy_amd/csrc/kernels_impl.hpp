@@ -487,37 +487,6 @@ __device__ inline void align_loop(const AlignArgs& a) {
       qnv = rows >= 32 ? 0u : ~((1u << rows) - 1u);
     }
 
-    if constexpr (S == 64 && W >= 2) {
-      // one pair per wave: the query base of a row is wave-uniform and rows
-      // past the query end are skipped. With 6 or more words per lane each
-      // row branches (scalar) to the code specialised for its base; with
-      // fewer the base enters as uniform masks (measured faster there: the
-      // branch costs more scalar issue than the two bit operations per word
-      // it saves).
-      const uint32_t qls = __builtin_amdgcn_readfirstlane(qlo);
-      const uint32_t qhs = __builtin_amdgcn_readfirstlane(qhi);
-      const int nrows = __builtin_amdgcn_readfirstlane(active ? min(32, mq - jb * 32) : 0);
-#pragma unroll 1
-      for (int r = 0; r < nrows; ++r) {
-        uint32_t s[W], u[W], carry;
-        if constexpr (W >= 6) {
-          // keeps the four match masks from being hoisted out of the row
-          // loop (they would take 4*W more registers)
-#pragma unroll
-          for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
-          const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
-          if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
-          else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
-          else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
-          else row_sum_base<3, W>(v, tl, th, s, u, carry);
-        } else {
-          const uint32_t ql = ((qls >> r) & 1u) ? ~0u : 0u;
-          const uint32_t qh = ((qhs >> r) & 1u) ? ~0u : 0u;
-          row_sum_masks<W>(v, tl, th, ql, qh, s, u, carry);
-        }
-        row_finish64<W>(v, s, u, carry, lane);
-      }
-    } else {
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
     for (int r0 = 0; r0 < 32; r0 += UR) {
@@ -551,7 +520,6 @@ __device__ inline void align_loop(const AlignArgs& a) {
           v[w] = t | (v[w] & x[w]);
         }
       }
-    }
     }
 
     // ---- block end: exact early decisions on the zero count
@@ -606,9 +574,311 @@ __device__ inline void align_loop(const AlignArgs& a) {
   }
 }
 
+// ------------------------------------------------ 64-lane tiles, W >= 2
+//
+// One pair per wave, so every row is wave-uniform. On top of the banded
+// recurrence above, the window follows the band's left edge as the DP values
+// sharpen it, and its width follows the band:
+//
+// * Left edge from the DP values. After row j (R rows left) a path that
+//   leaves row j at text column i ends with at most D[j][i] + min(R, n-i)
+//   matches, D[j][i] being the zero count of columns < i. D is
+//   non-decreasing in i, so the columns whose D + R < c form a prefix. A path
+//   of at least c matches keeps its prefix inside the window (induction over
+//   the blocks), so its value at its row-j column is at least its prefix
+//   length and that column is not in the prefix: the prefix columns can be
+//   frozen like the columns left of the diagonal band. The same bound gives
+//   the reject test D[j][n-R] + R < c (the maximum over i sits at i = n-R),
+//   never weaker than z + R < c.
+// * Width. At each 32-row block end the window is rebased at the first live
+//   word and given the narrowest width of the list (2, 3, 4, 6, 8, 10, 12, 16
+//   words per lane, at most the tile's) that reaches the band's right edge at
+//   the last row of the next block; the content moves through LDS. The tile's
+//   own width always suffices (it covers the whole diagonal band). Words
+//   entering on the right start all ones; words dropped on the right lie
+//   right of the band's right edge, so on no band path yet, and re-enter all
+//   ones, a state no larger than the true one: every value stays a lower
+//   bound of the LCS while every band path stays inside the window, so the
+//   decisions stay exact.
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+struct Pair64 {
+  const uint2* pk;
+  int64_t toff, qoff;  // first packed word of the bit-vector (text) and row (query) sequences
+  int n, mq, c, ntw;   // text length, row count, cutoff + 1, text words
+};
+
+// Per-wave LDS staging used to re-lay a window.
+struct Stage64 {
+  uint32_t* v;
+  uint32_t* tl;
+  uint32_t* th;
+};
+
+// Narrowest width of the list (at most wmax) whose window based at word
+// `base` reaches word `last`.
+__device__ inline int width_to(int base, int last, int wmax) {
+  const int ws[8] = {2, 3, 4, 6, 8, 10, 12, 16};
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (ws[i] >= wmax || base + 64 * ws[i] > last) return min(ws[i], wmax);
+  return wmax;
+}
+
+// Loads a window of W words per lane based at word b. Words that were in
+// the staged window (based at ob, ow words per lane) come from LDS, the rest
+// start all ones with their text from the store.
+template <int W>
+__device__ __forceinline__ void load_window64(const Pair64& P, const Stage64& X, int lane, int b, int ob, int ow,
+                                              uint32_t (&v)[W], uint32_t (&tl)[W], uint32_t (&th)[W]) {
+  // selects rather than branches: writing the word arrays in divergent
+  // branches would move them to scratch memory
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int g = b + lane * W + w;
+    const int rel = g - ob;
+    const bool staged = rel < 64 * ow;
+    const int ri = staged ? rel : 0;
+    const uint32_t sv = X.v[ri], sl = X.tl[ri], sh = X.th[ri];
+    uint2 t = make_uint2(0u, 0u);
+    if (!staged && g < P.ntw) t = P.pk[P.toff + g];
+    v[w] = staged ? sv : ~0u;
+    tl[w] = staged ? sl : t.x;
+    th[w] = staged ? sh : t.y;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int W>
+__device__ __forceinline__ void stage_window64(const Stage64& X, int lane, const uint32_t (&v)[W],
+                                               const uint32_t (&tl)[W], const uint32_t (&th)[W]) {
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    X.v[lane * W + w] = v[w];
+    X.tl[lane * W + w] = tl[w];
+    X.th[lane * W + w] = th[w];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Window state of a pair between width changes.
+struct Win64 {
+  int b, jb, frozen;  // window base word, next 32-row block, zeros left of the window
+  int ob, ow;         // staged window (base word, words per lane; ow = 0: none)
+  int w;              // width to run next
+  int w0;             // widest width (the tile's)
+};
+
+// Runs 32-row blocks of the pair with W words per lane, from the state in
+// S (window contents staged in X). Returns 1 (accept) or 0 (reject), or -1
+// after staging the window when the next block needs another width (S.w).
+template <int W>
+__device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, int lane, Win64& S,
+                                            unsigned long long& cells, unsigned long long& cdec) {
+  const int n = P.n, mq = P.mq, c = P.c;
+  int b = S.b, jb = S.jb, frozen = S.frozen;
+  uint32_t v[W], tl[W], th[W];
+  load_window64<W>(P, X, lane, b, S.ob, S.ow, v, tl, th);
+  while (true) {
+    // ---- the 32 rows of block jb (fewer in the last block)
+    const uint2 qw = P.pk[P.qoff + jb];
+    const uint32_t qls = __builtin_amdgcn_readfirstlane(qw.x);
+    const uint32_t qhs = __builtin_amdgcn_readfirstlane(qw.y);
+    const int nrows = min(32, mq - jb * 32);
+#pragma unroll 1
+    for (int r = 0; r < nrows; ++r) {
+      uint32_t s[W], u[W], carry;
+      if constexpr (W >= 6) {
+        // With 6 or more words per lane each row branches (scalar) to the
+        // code specialised for its base; the empty asm keeps the four match
+        // masks from being hoisted out of the loop (4*W more registers).
+#pragma unroll
+        for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
+        const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
+        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
+        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
+        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
+        else row_sum_base<3, W>(v, tl, th, s, u, carry);
+      } else {
+        // with fewer words the base enters as uniform masks: measured faster
+        // there than the branch, whose scalar issue outweighs the two bit
+        // operations per word it saves
+        const uint32_t ql = ((qls >> r) & 1u) ? ~0u : 0u;
+        const uint32_t qh = ((qhs >> r) & 1u) ? ~0u : 0u;
+        row_sum_masks<W>(v, tl, th, ql, qh, s, u, carry);
+      }
+      row_finish64<W>(v, s, u, carry, lane);
+    }
+    if (lane == 0) cells += 64ull * W * 32ull * 32ull;
+
+    // ---- block end: zero counts per lane (all columns, and columns < n-R)
+    const int jl = jb * 32 + nrows - 1;  // last row done
+    const int R = mq - 1 - jl;           // rows left
+    const int istar = max(0, n - R);
+    uint32_t x = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const int g = b + lane * W + w;
+      x += (uint32_t)__popc(~v[w] & valid_bits(g, n)) | ((uint32_t)__popc(~v[w] & valid_bits(g, istar)) << 16);
+    }
+    uint32_t tot = x;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    tot = __builtin_amdgcn_readfirstlane(tot);
+    const int z = frozen + (int)(tot & 0xffffu);
+    // columns < n-R left of the window are all counted: an upper bound
+    const int dstar = frozen + (int)(tot >> 16);
+    if (z >= c || dstar + R < c || R == 0) {
+      if (lane == 0) cdec += (unsigned long long)band_cells_rows(n, mq, c, jl + 1);
+      return z >= c ? 1 : 0;
+    }
+    ++jb;
+
+    // ---- window for the next block: from the first live word, reaching
+    // the band's right edge at the block's last row. The DP edge (a prefix
+    // scan of the lane counts) is taken every few blocks, or when the
+    // window must move anyway; otherwise the diagonal edge alone is used
+    // (dropping fewer columns is always exact).
+    const int jn = jb * 32;
+    const int hi_col = min(n - 1, jn + 31 + n - c);  // right edge at the last row of the next block
+    const int last = hi_col >> 5;
+    int lo_col = max(0, jn - (mq - c));
+    const bool covered = b + 64 * W > last;
+    constexpr int kEvery = W >= 6 ? 2 : 4;
+    if (!covered || (W > 2 && jb % kEvery == 0)) {
+      // inclusive prefix over the lanes: D at the end of each lane's words;
+      // lanes whose last column has D + R < c are dead (a prefix of lanes)
+      uint32_t pre = x & 0xffffu;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(pre, d);
+        if (lane >= d) pre += y;
+      }
+      const bool dead = frozen + (int)pre + R < c;
+      const int ndead = __popcll(__builtin_amdgcn_ballot_w64(dead));
+      const int dead_col = (b + ndead * W) * 32;  // columns before it are dead after row jl
+      if (dead_col > min(n - 1, jl + n - c)) {
+        // every column of row jl's band is dead: no path of c matches exists
+        if (lane == 0) cdec += (unsigned long long)band_cells_rows(n, mq, c, jl + 1);
+        return 0;
+      }
+      lo_col = max(lo_col, dead_col);
+      const int nb = max(b, lo_col >> 5);
+      const int w2 = width_to(nb, last, S.w0);
+      if (w2 < W || !covered) {
+        // zeros of the words dropped on the left
+        int dz = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const int g = b + lane * W + w;
+          if (g < nb) dz += __popc(~v[w] & valid_bits(g, n));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dz += __shfl_xor(dz, off);
+        stage_window64<W>(X, lane, v, tl, th);
+        frozen += uniform(dz);
+        if (w2 != W) {
+          S.b = nb; S.jb = jb; S.frozen = frozen; S.ob = b; S.ow = W; S.w = w2;
+          return -1;
+        }
+        load_window64<W>(P, X, lane, nb, b, W, v, tl, th);
+        b = nb;
+      }
+    }
+  }
+}
+
+// Decides LCS >= c for a pair on a 64-lane tile of at most W words per lane.
+template <int W>
+__device__ __forceinline__ int decide64(const Pair64& P, const Stage64& X, int lane, unsigned long long& cells,
+                                        unsigned long long& cdec) {
+  Win64 S{0, 0, 0, 0, 0, W, W};
+  while (true) {
+    int r = -1;
+    switch (S.w) {
+      case 16: if constexpr (W >= 16) r = run_blocks64<16>(P, X, lane, S, cells, cdec); break;
+      case 12: if constexpr (W >= 12) r = run_blocks64<12>(P, X, lane, S, cells, cdec); break;
+      case 10: if constexpr (W >= 10) r = run_blocks64<10>(P, X, lane, S, cells, cdec); break;
+      case 8: if constexpr (W >= 8) r = run_blocks64<8>(P, X, lane, S, cells, cdec); break;
+      case 6: if constexpr (W >= 6) r = run_blocks64<6>(P, X, lane, S, cells, cdec); break;
+      case 4: if constexpr (W >= 4) r = run_blocks64<4>(P, X, lane, S, cells, cdec); break;
+      case 3: if constexpr (W >= 3) r = run_blocks64<3>(P, X, lane, S, cells, cdec); break;
+      default: r = run_blocks64<2>(P, X, lane, S, cells, cdec); break;
+    }
+    if (r >= 0) return r;
+  }
+}
+
+template <int W>
+__device__ inline void align_loop64(const AlignArgs& a) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  __shared__ uint32_t stage[4][3][64 * W];
+  const Stage64 X{stage[wid][0], stage[wid][1], stage[wid][2]};
+  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
+  const int total = *a.count;
+  while (true) {
+    int got = 0;
+    if (lane == 0) got = atomicAdd(a.work, 1);
+    got = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
+    if (got >= total) break;
+    // the pair's scalars are wave-uniform: reading them through the first
+    // lane keeps them (and the control flow that depends on them) scalar
+    const AlignItem it = a.items[got];
+    const int text = uniform(it.text), query = uniform(it.query), jobi = uniform(it.job), rslot = uniform(it.slot);
+    const JobDev& J = a.jobs[jobi];
+    const StoreView& st = J.st;
+    const int islot = rslot & (kSwapFlag - 1);
+    const int cand = (rslot & kSwapFlag) ? text : query;  // the candidate owns the cutoff
+    const int n = uniform(st.m[text]), mq = uniform(st.m[query]), c = uniform(st.bc[cand]) + 1;
+    if (a.skip_absorbed && uniform64(J.acc[cand] & J.state->tmask & ((1ull << islot) - 1ull))) {
+      if (lane == 0) ++nskip;
+      continue;
+    }
+    int res = 0;
+    if (c <= min(n, mq)) {
+      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1};
+      res = decide64<W>(P, X, lane, cells, cdec);
+    }
+    if (lane == 0) {
+      if (a.result) {
+        a.result[got] = res;
+      } else if (res) {
+        atomicOr(&J.acc[cand], 1ull << islot);
+      }
+      if (res) ++nacc; else ++nrej;
+    }
+  }
+  if (lane == 0 && a.stats) {
+    if (cells) atomicAdd(&a.stats[ST_CELLS_EXEC], cells);
+    if (cdec) atomicAdd(&a.stats[ST_CELLS_DECIDED], cdec);
+    if (nskip) atomicAdd(&a.stats[ST_SPEC_WASTE], nskip);
+    if (nacc) atomicAdd(&a.stats[ST_ACCEPT], nacc);
+    if (nrej) atomicAdd(&a.stats[ST_REJ_ALIGN], nrej);
+  }
+}
+
+// Waves per SIMD the register allocation must allow (HIP's second launch
+// bound): 4 up to 10 words per lane, then 3 and 2 (measured: the widest
+// tiles cannot fit more without spilling).
+__host__ __device__ constexpr int align_waves_per_eu(int s, int w) {
+  return s < 64 || w <= 10 ? 4 : w <= 12 ? 3 : 2;
+}
+
 template <int S, int W>
-__global__ __launch_bounds__(256) void align_kernel(AlignArgs a) {
-  align_loop<S, W>(a);
+__global__ __launch_bounds__(256, align_waves_per_eu(S, W)) void align_kernel(AlignArgs a) {
+  if constexpr (S == 64 && W >= 2) {
+    align_loop64<W>(a);
+  } else {
+    align_loop<S, W>(a);
+  }
 }
 
 #endif  // GSDR_DEVICE_TU
