@@ -202,17 +202,30 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
 }
 
 // -------------------------------------------------------------- launchers
-hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
-  const size_t lds = sizeof(uint16_t) << (2 * max_k);
+template <int G>
+static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+  const size_t lds = (size_t)G * (sizeof(uint16_t) << (2 * max_k));
   static bool attr_set = false;
   if (lds > 32768 && !attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(uint16_t) << 16));
+    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(uint16_t) << 16));
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(filter_kernel<G>, dim3(grid), dim3(256), lds, s, a);
   return hipGetLastError();
+}
+
+// The group size the kernel uses is filter_group(max_k); phase_units and
+// every job's stored profiles must be for k <= max_k.
+hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+  switch (filter_group(max_k)) {
+    case 16: return launch_filter_g<16>(a, max_k, grid, s);
+    case 8: return launch_filter_g<8>(a, max_k, grid, s);
+    case 4: return launch_filter_g<4>(a, max_k, grid, s);
+    case 2: return launch_filter_g<2>(a, max_k, grid, s);
+    default: return launch_filter_g<1>(a, max_k, grid, s);
+  }
 }
 
 template <int S, int W>

@@ -106,13 +106,29 @@ struct JobState {
 // other (slots 0..k_spec-1), B = true representatives against the rest.
 enum FilterPhase { PHASE_REPS = 0, PHASE_MAIN = 1 };
 
-// Work units of a job in a phase (kFilterChunk candidate positions each).
-__host__ __device__ inline int phase_units(const JobState& s, int phase) {
+// Work units of a job in a phase: (group of up to `group` representative
+// slots, kFilterChunk candidate positions). Phase A: slots 0..K-1 against
+// positions 0..K-1; phase B: the kt true representatives against positions
+// K..nc-1.
+__host__ __device__ inline int phase_units(const JobState& s, int phase, int group) {
   if (s.done) return 0;
   const int K = s.k_spec;
-  if (phase == PHASE_REPS) return K > 1 ? K * ((K + kFilterChunk - 1) / kFilterChunk) : 0;
+  if (phase == PHASE_REPS)
+    return K > 1 ? ((K + group - 1) / group) * ((K + kFilterChunk - 1) / kFilterChunk) : 0;
   const int rest = s.nc - K;
-  return rest > 0 ? s.kt * ((rest + kFilterChunk - 1) / kFilterChunk) : 0;
+  return rest > 0 ? ((s.kt + group - 1) / group) * ((rest + kFilterChunk - 1) / kFilterChunk) : 0;
+}
+
+// Representatives whose shared-word tables a filter block holds at once:
+// as many as fit 32 KiB of LDS (u16 counts over 4^k words), at most 16.
+__host__ __device__ inline int filter_group(int k) {
+  const int bytes = 2 << (2 * k);
+  int g = 32768 / bytes;
+  if (g >= 16) return 16;
+  if (g >= 8) return 8;
+  if (g >= 4) return 4;
+  if (g >= 2) return 2;
+  return 1;
 }
 
 // One clustering job as the kernels see it.

@@ -85,20 +85,28 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
 // GSDR_DEVICE_TU (kernels.hip); the other files see the launchers.
 #ifdef GSDR_DEVICE_TU
 // ------------------------------------------------------------------ filter
-// Work unit = (job, representative slot i, chunk of kFilterChunk candidate
-// positions). Each 16-lane group owns one candidate q at a position after
-// slot i:
+// Work unit = (job, group of up to G representative slots, chunk of
+// kFilterChunk candidate positions). A 256-thread block stages, for the G
+// representatives of its unit, the k-mer counts c_r(w) as u16 tables of 4^k
+// entries in LDS plus their id, length and composition (rebuilt only when
+// the unit's group changes). Each 16-lane group owns one candidate q at a
+// position after each representative's slot and reads q's record and
+// distinct k-mer profile once for the whole group:
 //   length check (added, sound): c = bc_q + 1 > min(n, m_q) means LCS < c;
 //   composition (func.cpp:525-543): reject sum < bc_q, plus the sound
 //     tightening sum == bc_q (LCS <= sum < bc_q + 1);
 //   shared words (func.cpp:544-571): sum over q's distinct k-mers of
-//     min(c_r(w), c_q(w)) < wc_q rejects, with c_r staged in LDS.
+//     min(c_r(w), c_q(w)) < wc_q rejects (one LDS lookup per representative
+//     of the group that passed the checks above).
 // Survivors are appended to the alignment list of their tile shape.
+template <int G>
 __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
   HIP_DYNAMIC_SHARED(uint32_t, lds)
   uint16_t* table = reinterpret_cast<uint16_t*>(lds);
   __shared__ int s_pre[kMaxJobsPerLaunch + 1];
   __shared__ int s_part[4];
+  __shared__ int s_slot[G], s_rep[G], s_n[G];
+  __shared__ uint4 s_comp[G];
   const int tid = threadIdx.x;
   const int nj = a.njobs;
   // prefix of work units over jobs
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     for (int base = 0; base < nj; base += 256) {
       const int j = base + tid;
       int u = 0;
-      if (j < nj) u = phase_units(*a.jobs[j].state, a.phase);
+      if (j < nj) u = phase_units(*a.jobs[j].state, a.phase, G);
       int tot;
       const int ex = block_exclusive_scan<256>(u, &tot, s_part);
       if (j < nj) s_pre[j] = carry + ex;
@@ -122,8 +130,8 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
   const int per = (total + gridDim.x - 1) / gridDim.x;
   const int u0 = blockIdx.x * per;
   const int u1 = min(total, u0 + per);
-  const int g = tid >> 4, gl = tid & 15;
-  int cj = -1, cs = -1;
+  const int grp = tid >> 4, gl = tid & 15;
+  int cj = -1, cg = -1, gcount = 0;
   unsigned long long rej_len = 0, rej_comp = 0, rej_word = 0, passed = 0, bytes = 0, cband = 0, cfull = 0;
   for (int u = u0; u < u1; ++u) {
     // job of unit u: last j with s_pre[j] <= u
@@ -138,92 +146,132 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     const int nc = S->nc;
     const int K = S->k_spec;
     const int32_t* cand = J.cand[S->cur];
-    // phase A: slots 0..K-1 against positions 0..K-1; phase B: the rank-th
-    // true representative against positions K..nc-1
     const int p0 = a.phase == PHASE_REPS ? 0 : K;
     const int p1 = a.phase == PHASE_REPS ? K : nc;
+    const int members = a.phase == PHASE_REPS ? K : S->kt;
     const int chunks = (p1 - p0 + kFilterChunk - 1) / kFilterChunk;
     const int lu = u - s_pre[job];
-    const int rank = lu / chunks;
-    const int chunk = lu - rank * chunks;
-    int slot = rank;
-    if (a.phase == PHASE_MAIN) {
-      unsigned long long tm = S->tmask;
-      for (int i = 0; i < rank; ++i) tm &= tm - 1;
-      slot = __builtin_ctzll(tm);
-    }
-    const int r = cand[slot];
-    if (job != cj || slot != cs) {
+    const int gi = lu / chunks;
+    const int chunk = lu - gi * chunks;
+    if (job != cj || gi != cg) {
+      // stage the group: slots (phase A: ranks are slots; phase B: the
+      // rank-th true representative), records and k-mer tables
       const int nw = 1 << (2 * J.k);
+      gcount = min(G, members - gi * G);
       __syncthreads();
-      for (int i = tid; i < nw / 2; i += 256) lds[i] = 0u;
+      for (int i = tid; i < G * nw / 2; i += 256) lds[i] = 0u;
+      if (tid < gcount) {
+        const int rank = gi * G + tid;
+        int slot = rank;
+        if (a.phase == PHASE_MAIN) {
+          unsigned long long tm = S->tmask;
+          for (int i = 0; i < rank; ++i) tm &= tm - 1;
+          slot = __builtin_ctzll(tm);
+        }
+        const int r = cand[slot];
+        s_slot[tid] = slot;
+        s_rep[tid] = r;
+        s_n[tid] = J.st.m[r];
+        s_comp[tid] = J.st.comp[r];
+      }
       __syncthreads();
-      const int64_t k0 = J.st.kpoff[r], k1 = J.st.kpoff[r + 1];
-      for (int64_t e = k0 + tid; e < k1; e += 256) {
-        const uint32_t x = J.st.kp[e];
-        table[x & 0xFFFFu] = (uint16_t)(x >> 16);
+      for (int g = 0; g < gcount; ++g) {
+        const int r = s_rep[g];
+        const int64_t k0 = J.st.kpoff[r], k1 = J.st.kpoff[r + 1];
+        uint16_t* tg = table + g * nw;
+        for (int64_t e = k0 + tid; e < k1; e += 256) {
+          const uint32_t x = J.st.kp[e];
+          tg[x & 0xFFFFu] = (uint16_t)(x >> 16);
+        }
       }
       __syncthreads();
       cj = job;
-      cs = slot;
+      cg = gi;
     }
-    const int pos = p0 + chunk * kFilterChunk + g;
-    if (pos <= slot || pos >= p1) continue;
+    const int pos = p0 + chunk * kFilterChunk + grp;
+    if (pos >= p1) continue;
     const int q = cand[pos];
-    const int n = J.st.m[r];
     const int mq = J.st.m[q];
     const int bcq = J.st.bc[q];
     const int c = bcq + 1;
+    const uint4 cq = J.st.comp[q];
     if (gl == 0) bytes += 28;  // id, m, bc, comp
-    if (c > min(n, mq)) {
-      if (gl == 0) ++rej_len;
-      continue;
+    // length and composition per representative of the group
+    uint32_t mask = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (g >= gcount || pos <= s_slot[g]) continue;
+      if (c > min(s_n[g], mq)) {
+        if (gl == 0) ++rej_len;
+        continue;
+      }
+      const uint4 cr = s_comp[g];
+      const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
+      if (sum <= bcq) {
+        if (gl == 0) ++rej_comp;
+        continue;
+      }
+      mask |= 1u << g;
     }
-    const uint4 cr = J.st.comp[r], cq = J.st.comp[q];
-    const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
-    if (sum <= bcq) {
-      if (gl == 0) ++rej_comp;
-      continue;
-    }
+    if (mask == 0) continue;
+    // shared words: q's profile read once, one table lookup per survivor so far
+    const int nw = 1 << (2 * J.k);
     const int64_t e0 = J.st.kpoff[q], e1 = J.st.kpoff[q + 1];
-    int acc = 0;
+    int acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = 0;
     int64_t e = e0 + gl;
     for (; e + 48 < e1; e += 64) {
       const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
-      acc += min((uint32_t)table[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)table[x1 & 0xFFFFu], x1 >> 16) +
-             min((uint32_t)table[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)table[x3 & 0xFFFFu], x3 >> 16);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (!((mask >> g) & 1u)) continue;
+        const uint16_t* tg = table + g * nw;
+        acc[g] += min((uint32_t)tg[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)tg[x1 & 0xFFFFu], x1 >> 16) +
+                  min((uint32_t)tg[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)tg[x3 & 0xFFFFu], x3 >> 16);
+      }
     }
     for (; e < e1; e += 16) {
       const uint32_t x = J.st.kp[e];
-      acc += min((uint32_t)table[x & 0xFFFFu], x >> 16);
-    }
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
-    if (gl == 0) {
-      bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
-      if (acc < J.st.wc[q]) {
+      for (int g = 0; g < G; ++g) {
+        if (!((mask >> g) & 1u)) continue;
+        acc[g] += min((uint32_t)table[g * nw + (x & 0xFFFFu)], x >> 16);
+      }
+    }
+    if (gl == 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
+    const int wcq = J.st.wc[q];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (!((mask >> g) & 1u)) continue;
+      int sg = acc[g];
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) sg += __shfl_xor(sg, off, 16);
+      if (gl != 0) continue;
+      if (sg < wcq) {
         ++rej_word;
+        continue;
+      }
+      const int r = s_rep[g], slot = s_slot[g], n = s_n[g];
+      ++passed;
+      bytes += 16;
+      cband += (unsigned long long)band_cells(n, mq, c);
+      cfull += (unsigned long long)n * (unsigned long long)mq;
+      const bool sw = swap_roles(n, mq);
+      const int sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
+      const int at = atomicAdd(&a.bin_count[sh], 1);
+      AlignItem it;
+      it.text = sw ? q : r;
+      it.query = sw ? r : q;
+      it.job = job;
+      it.slot = slot | (sw ? kSwapFlag : 0);
+      if (at < a.item_cap) {
+        a.items[(int64_t)sh * a.item_cap + at] = it;
       } else {
-        ++passed;
-        bytes += 16;
-        cband += (unsigned long long)band_cells(n, mq, c);
-        cfull += (unsigned long long)n * (unsigned long long)mq;
-        const bool sw = swap_roles(n, mq);
-        const int sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
-        const int at = atomicAdd(&a.bin_count[sh], 1);
-        AlignItem it;
-        it.text = sw ? q : r;
-        it.query = sw ? r : q;
-        it.job = job;
-        it.slot = slot | (sw ? kSwapFlag : 0);
-        if (at < a.item_cap) {
-          a.items[(int64_t)sh * a.item_cap + at] = it;
-        } else {
-          // the list is full: count it so the host fails the call (the host
-          // bounds k_spec * nc so that this does not happen)
-          atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
-          atomicSub(&a.bin_count[sh], 1);
-        }
+        // the list is full: count it so the host fails the call (the host
+        // bounds k_spec * nc so that this does not happen)
+        atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
+        atomicSub(&a.bin_count[sh], 1);
       }
     }
   }
