@@ -555,11 +555,22 @@ __device__ inline void align_loop(const AlignArgs& a) {
           carry = co;
           pall &= s[w];
         }
-        const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
-        const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
-        const uint64_t X = G | P;
-        const uint64_t C = (X + G) ^ X ^ G;
-        uint32_t cin = (uint32_t)(C >> lane) & 1u;
+        // A lane passes a carry on only when its whole sum is ones. When no
+        // lane of a live row has that, the carry into lane l is the carry
+        // out of lane l-1 (one lane shift; none into a segment's first
+        // lane). Rows past a query end and segments without a pair change
+        // nothing, so they do not count.
+        uint32_t cin;
+        if (__builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0) {
+          cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+          if (sl == 0) cin = 0u;
+        } else {
+          const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
+          const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+          const uint64_t X = G | P;
+          const uint64_t C = (X + G) ^ X ^ G;
+          cin = (uint32_t)(C >> lane) & 1u;
+        }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           uint32_t co;
@@ -570,21 +581,30 @@ __device__ inline void align_loop(const AlignArgs& a) {
       }
     }
 
-    // ---- block end: exact early decisions on the zero count
-    int zl = 0;
+    // ---- block end: exact early decisions on the zero counts of all
+    // columns (z) and of the columns < n-R (R rows left): accept z >= c,
+    // reject D[j][n-R] + R < c (columns left of the window all counted, an
+    // upper bound), never weaker than z + R < c
+    const int jl = min(jb * 32 + 31, mq - 1);
+    const int R = mq - 1 - jl;
+    const int istar = max(0, n - R);
+    uint32_t zx = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) zl += __popc(~v[w] & valid_bits(b + sl * W + w, n));
+    for (int w = 0; w < W; ++w) {
+      const int g = b + sl * W + w;
+      zx += (uint32_t)__popc(~v[w] & valid_bits(g, n)) | ((uint32_t)__popc(~v[w] & valid_bits(g, istar)) << 16);
+    }
 #pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) zl += __shfl_xor(zl, off);
+    for (int off = S / 2; off > 0; off >>= 1) zx += __shfl_xor(zx, off);
     if (active) {
-      const int z = frozen + zl;
-      const int jl = min(jb * 32 + 31, mq - 1);
+      const int z = frozen + (int)(zx & 0xffffu);
+      const int dstar = frozen + (int)(zx >> 16);
       bool accept = false, reject = false;
       if (hopeless) {
         reject = true;
       } else if (z >= c) {
         accept = true;
-      } else if (z + (mq - 1 - jl) < c) {
+      } else if (dstar + R < c) {
         reject = true;
       }
       if (sl == 0) cells += (unsigned long long)S * W * 32ull * 32ull;
