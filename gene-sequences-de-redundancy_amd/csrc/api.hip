@@ -111,7 +111,7 @@ int gsdr_records_write_fasta(const gsdr_records* r, const char* path, int32_t li
 int gsdr_records_sort(gsdr_records* r, int64_t* perm_out) {
   if (!r) return fail(GSDR_E_INVALID, "null records");
   std::vector<int64_t> perm = length_order(r->r);
-  r->r = permute(r->r, perm);
+  r->r = permute(r->r, perm, host_threads());
   if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
   return GSDR_OK;
 }

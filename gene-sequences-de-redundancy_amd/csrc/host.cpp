@@ -3,6 +3,7 @@
 #include "host.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -97,18 +98,36 @@ std::vector<int64_t> length_order(const Records& r) {
   return perm;
 }
 
-Records permute(const Records& r, const std::vector<int64_t>& perm) {
+Records permute(const Records& r, const std::vector<int64_t>& perm, int threads) {
+  const int64_t n = (int64_t)perm.size();
   Records o;
-  o.names.reserve(r.names.size());
-  o.data.reserve(r.data.size());
-  o.name_off.reserve(perm.size() + 1);
-  o.data_off.reserve(perm.size() + 1);
-  for (int64_t src : perm) {
-    o.names.insert(o.names.end(), r.names.begin() + r.name_off[src], r.names.begin() + r.name_off[src + 1]);
-    o.name_off.push_back((int64_t)o.names.size());
-    o.data.insert(o.data.end(), r.data.begin() + r.data_off[src], r.data.begin() + r.data_off[src + 1]);
-    o.data_off.push_back((int64_t)o.data.size());
+  o.name_off.assign((size_t)n + 1, 0);
+  o.data_off.assign((size_t)n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t src = perm[(size_t)i];
+    o.name_off[(size_t)i + 1] = o.name_off[(size_t)i] + (r.name_off[src + 1] - r.name_off[src]);
+    o.data_off[(size_t)i + 1] = o.data_off[(size_t)i] + (r.data_off[src + 1] - r.data_off[src]);
   }
+  o.names.resize((size_t)o.name_off[(size_t)n]);
+  o.data.resize((size_t)o.data_off[(size_t)n]);
+  if (threads < 1) threads = 1;
+  std::atomic<int64_t> next{0};
+  auto copy = [&]() {
+    constexpr int64_t kChunk = 1024;
+    for (int64_t lo; (lo = next.fetch_add(kChunk)) < n;) {
+      const int64_t hi = std::min(n, lo + kChunk);
+      for (int64_t i = lo; i < hi; ++i) {
+        const int64_t src = perm[(size_t)i];
+        std::memcpy(o.names.data() + o.name_off[(size_t)i], r.names.data() + r.name_off[src],
+                    (size_t)(r.name_off[src + 1] - r.name_off[src]));
+        std::memcpy(o.data.data() + o.data_off[(size_t)i], r.data.data() + r.data_off[src],
+                    (size_t)(r.data_off[src + 1] - r.data_off[src]));
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < threads; ++t) th.emplace_back(copy);
+  for (auto& x : th) x.join();
   return o;
 }
 
@@ -204,47 +223,65 @@ void preprocess(const Records& sorted, int k, int threads, Prep& p) {
   p.comp.assign((size_t)n * 4, 0);
   p.pk_off.assign((size_t)n + 1, 0);
   p.kp_off.assign((size_t)n + 1, 0);
-  // pass 1: lengths, gaps, composition, profile sizes need the data; do
-  // packing offsets from raw lengths (upper bound: words = m/32+1 <= L/32+1),
-  // but use the exact m so the store is compact.
   if (threads < 1) threads = 1;
+  // Pass 1: lengths, composition and the distinct k-mers of every sequence,
+  // kept in a buffer of the thread that made them. Sequences are handed out
+  // in chunks from a shared counter: the input is sorted by length, so
+  // fixed ranges would leave one thread all the long sequences.
+  constexpr int64_t kChunk = 256;
+  std::vector<std::vector<uint32_t>> tbuf((size_t)threads);
+  std::vector<int32_t> owner((size_t)n, 0);
+  std::vector<int64_t> at((size_t)n, 0);
   std::vector<int32_t> distinct((size_t)n, 0);
-  auto pass1 = [&](int64_t lo, int64_t hi) {
+  auto pass1 = [&](int tid, std::atomic<int64_t>* next) {
     const uint32_t nwords = 1u << (2 * k);
     std::vector<uint32_t> cnt(nwords, 0);
-    std::vector<uint16_t> touched;
-    for (int64_t s = lo; s < hi; ++s) {
-      const char* d = sorted.data.data() + sorted.data_off[s];
-      const int64_t L = sorted.len(s);
-      uint32_t c4[5] = {0, 0, 0, 0, 0};
-      for (int64_t i = 0; i < L; ++i) c4[base_code((unsigned char)d[i])]++;
-      p.raw_len[s] = (int32_t)L;
-      p.m[s] = (int32_t)(c4[0] + c4[1] + c4[2] + c4[3]);
-      for (int b = 0; b < 4; ++b) p.comp[s * 4 + b] = c4[b];
-      // distinct k-mers
-      touched.clear();
-      uint32_t word = 0, mask = nwords - 1;
-      int valid = 0;
-      for (int64_t i = 0; i < L; ++i) {
-        uint8_t c = base_code((unsigned char)d[i]);
-        if (c == 4) { valid = 0; word = 0; continue; }
-        word = ((word << 2) | c) & mask;
-        if (++valid >= k) {
-          if (cnt[word]++ == 0) touched.push_back((uint16_t)word);
+    std::vector<uint32_t> touched(nwords);
+    std::vector<uint32_t>& out = tbuf[(size_t)tid];
+    // composition as two registers of two 32-bit counts (A|C, G|T)
+    static const uint64_t kIncLo[5] = {1ull, 1ull << 32, 0, 0, 0};
+    static const uint64_t kIncHi[5] = {0, 0, 1ull, 1ull << 32, 0};
+    for (int64_t lo; (lo = next->fetch_add(kChunk)) < n;) {
+      const int64_t hi = std::min(n, lo + kChunk);
+      for (int64_t s = lo; s < hi; ++s) {
+        const unsigned char* d = (const unsigned char*)sorted.data.data() + sorted.data_off[s];
+        const int64_t L = sorted.len(s);
+        uint64_t clo = 0, chi = 0;
+        uint32_t nt = 0, word = 0;
+        const uint32_t mask = nwords - 1;
+        int valid = 0;
+        for (int64_t i = 0; i < L; ++i) {
+          const uint8_t c = base_code(d[i]);
+          clo += kIncLo[c];
+          chi += kIncHi[c];
+          if (c == 4) { valid = 0; word = 0; continue; }
+          word = ((word << 2) | c) & mask;
+          if (++valid >= k) {
+            if (cnt[word]++ == 0) touched[nt++] = word;
+          }
+        }
+        const uint32_t ca = (uint32_t)clo, cc = (uint32_t)(clo >> 32), cg = (uint32_t)chi, ct = (uint32_t)(chi >> 32);
+        p.raw_len[s] = (int32_t)L;
+        p.m[s] = (int32_t)(ca + cc + cg + ct);
+        p.comp[s * 4 + 0] = ca;
+        p.comp[s * 4 + 1] = cc;
+        p.comp[s * 4 + 2] = cg;
+        p.comp[s * 4 + 3] = ct;
+        owner[s] = tid;
+        at[s] = (int64_t)out.size();
+        distinct[s] = (int32_t)nt;
+        for (uint32_t e = 0; e < nt; ++e) {
+          const uint32_t w = touched[e];
+          out.push_back(w | ((cnt[w] & 0xFFFFu) << 16));
+          cnt[w] = 0;
         }
       }
-      distinct[s] = (int32_t)touched.size();
-      for (uint16_t w : touched) cnt[w] = 0;
     }
   };
   auto run = [&](auto fn) {
+    std::atomic<int64_t> next{0};
     std::vector<std::thread> th;
-    int64_t chunk = (n + threads - 1) / threads;
-    for (int t = 0; t < threads; ++t) {
-      int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
-      if (lo >= hi) break;
-      th.emplace_back(fn, lo, hi);
-    }
+    for (int t = 0; t < threads; ++t) th.emplace_back(fn, t, &next);
     for (auto& x : th) x.join();
   };
   run(pass1);
@@ -255,39 +292,38 @@ void preprocess(const Records& sorted, int k, int threads, Prep& p) {
     p.max_m = std::max(p.max_m, p.m[s]);
   }
   p.pk.assign((size_t)p.pk_off[n] * 2, 0u);
-  p.kp.assign((size_t)p.kp_off[n], 0u);
-  auto pass2 = [&](int64_t lo, int64_t hi) {
-    const uint32_t nwords = 1u << (2 * k);
-    std::vector<uint32_t> cnt(nwords, 0);
-    std::vector<uint16_t> touched;
-    for (int64_t s = lo; s < hi; ++s) {
-      const char* d = sorted.data.data() + sorted.data_off[s];
-      const int64_t L = sorted.len(s);
-      uint32_t* w2 = p.pk.data() + p.pk_off[s] * 2;
-      int64_t j = 0;
-      for (int64_t i = 0; i < L; ++i) {
-        uint8_t c = base_code((unsigned char)d[i]);
-        if (c == 4) continue;
-        w2[(j >> 5) * 2 + 0] |= (uint32_t)(c & 1) << (j & 31);
-        w2[(j >> 5) * 2 + 1] |= (uint32_t)(c >> 1) << (j & 31);
-        ++j;
-      }
-      touched.clear();
-      uint32_t word = 0, mask = nwords - 1;
-      int valid = 0;
-      for (int64_t i = 0; i < L; ++i) {
-        uint8_t c = base_code((unsigned char)d[i]);
-        if (c == 4) { valid = 0; word = 0; continue; }
-        word = ((word << 2) | c) & mask;
-        if (++valid >= k) {
-          if (cnt[word]++ == 0) touched.push_back((uint16_t)word);
+  p.kp.resize((size_t)p.kp_off[n]);
+  // Pass 2: the gap-free bases as two bit planes, and the profiles moved to
+  // their place in sorted order.
+  auto pass2 = [&](int, std::atomic<int64_t>* next) {
+    for (int64_t lo; (lo = next->fetch_add(kChunk)) < n;) {
+      const int64_t hi = std::min(n, lo + kChunk);
+      for (int64_t s = lo; s < hi; ++s) {
+        const char* d = sorted.data.data() + sorted.data_off[s];
+        const int64_t L = sorted.len(s);
+        uint32_t* w2 = p.pk.data() + p.pk_off[s] * 2;
+        // both planes of 32 gap-free bases built in registers, stored once
+        uint32_t plo = 0, phi = 0;
+        int j = 0;
+        for (int64_t i = 0; i < L; ++i) {
+          const uint32_t c = base_code((unsigned char)d[i]);
+          if (c == 4) continue;
+          plo |= (c & 1u) << j;
+          phi |= (c >> 1) << j;
+          if (++j == 32) {
+            w2[0] = plo;
+            w2[1] = phi;
+            w2 += 2;
+            plo = phi = 0;
+            j = 0;
+          }
         }
-      }
-      uint32_t* kp = p.kp.data() + p.kp_off[s];
-      for (size_t e = 0; e < touched.size(); ++e) {
-        uint16_t w = touched[e];
-        kp[e] = (uint32_t)w | ((cnt[w] & 0xFFFFu) << 16);
-        cnt[w] = 0;
+        if (j > 0) {
+          w2[0] = plo;
+          w2[1] = phi;
+        }
+        const uint32_t* src = tbuf[(size_t)owner[s]].data() + at[s];
+        std::memcpy(p.kp.data() + p.kp_off[s], src, sizeof(uint32_t) * (size_t)distinct[s]);
       }
     }
   };

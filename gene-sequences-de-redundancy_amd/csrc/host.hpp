@@ -37,7 +37,7 @@ int read_fasta(const char* path, Records& out, std::string& err);
 // Stable-free length order of the reference (func.cpp:105-107): libstdc++
 // std::sort with "longer first". Returns perm with perm[sorted] = original.
 std::vector<int64_t> length_order(const Records& r);
-Records permute(const Records& r, const std::vector<int64_t>& perm);
+Records permute(const Records& r, const std::vector<int64_t>& perm, int threads = 1);
 
 // Writes representatives (cluster[s] == s) in sorted order, unwrapped
 // (func.cpp:785-796). Returns the number written, or -1 on I/O error.

@@ -220,7 +220,22 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     int acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) acc[g] = 0;
+    // the profile loads are latency bound: 16, then 4, in flight per lane
     int64_t e = e0 + gl;
+    for (; e + 15 * 16 < e1; e += 16 * 16) {
+      uint32_t xs[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) xs[i] = J.st.kp[e + 16 * i];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (!((mask >> g) & 1u)) continue;
+        const uint16_t* tg = table + g * nw;
+        int sg = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
+        acc[g] += sg;
+      }
+    }
     for (; e + 48 < e1; e += 64) {
       const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
 #pragma unroll
