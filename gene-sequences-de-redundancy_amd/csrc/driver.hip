@@ -27,37 +27,27 @@ static int dev_alloc(std::vector<void*>& allocs, int64_t& footprint, T** p, size
   return GSDR_OK;
 }
 
+static int store_alloc(gsdr_store& s, void** p, size_t bytes) {
+  return dev_alloc(s.allocs, s.footprint, (uint8_t**)p, bytes);
+}
+
+// The sequence arrays (pk, pkoff, m, comp, kp, kpoff) are built on the device
+// from the raw bytes (prep.hip); the round state is allocated here.
 int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
   HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
-  Prep p;
-  preprocess(sorted, k, host_threads(), p);
-  const int64_t n = p.n;
+  const int64_t n = sorted.count();
   s.n = (int32_t)n;
   s.k = k;
   s.bases = (int64_t)sorted.data.size();
-  s.max_m = p.max_m;
-  s.raw_len = p.raw_len;
-  s.m = p.m;
+  int st = store_build_device(c, sorted, k, s, store_alloc);
+  if (st != GSDR_OK) return st;
   auto& A = s.allocs;
   auto& F = s.footprint;
-  int st;
-  if ((st = dev_alloc(A, F, &s.pk, (size_t)p.pk_off[n])) || (st = dev_alloc(A, F, &s.pkoff, (size_t)n + 1)) ||
-      (st = dev_alloc(A, F, &s.m_d, (size_t)n)) || (st = dev_alloc(A, F, &s.comp, (size_t)n)) ||
-      (st = dev_alloc(A, F, &s.kp, (size_t)p.kp_off[n])) || (st = dev_alloc(A, F, &s.kpoff, (size_t)n + 1)) ||
-      (st = dev_alloc(A, F, &s.bc, (size_t)n)) || (st = dev_alloc(A, F, &s.wc, (size_t)n)) ||
+  if ((st = dev_alloc(A, F, &s.bc, (size_t)n)) || (st = dev_alloc(A, F, &s.wc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.state, (size_t)1)))
     return st;
-  hipStream_t q = c->upload;
-  HIP_TRY(hipMemcpyAsync(s.pk, p.pk.data(), p.pk.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload pk");
-  HIP_TRY(hipMemcpyAsync(s.pkoff, p.pk_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, q), "upload pkoff");
-  HIP_TRY(hipMemcpyAsync(s.m_d, p.m.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, q), "upload m");
-  HIP_TRY(hipMemcpyAsync(s.comp, p.comp.data(), n * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload comp");
-  if (!p.kp.empty())
-    HIP_TRY(hipMemcpyAsync(s.kp, p.kp.data(), p.kp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, q), "upload kp");
-  HIP_TRY(hipMemcpyAsync(s.kpoff, p.kp_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, q), "upload kpoff");
-  HIP_TRY(hipStreamSynchronize(q), "upload sync");
   return GSDR_OK;
 }
 

@@ -96,6 +96,10 @@ int hip_fail(hipError_t e, const char* what);
 int host_threads();
 
 int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s);
+// Device-side build of the sequence arrays of a store (prep.hip); `alloc`
+// allocates device memory owned by the store.
+int store_build_device(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s,
+                       int (*alloc)(gsdr_store&, void**, size_t));
 void store_release(gsdr_store& s);
 int store_set_threshold(gsdr_store& s, float t);
 int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out);
