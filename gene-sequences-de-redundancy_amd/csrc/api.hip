@@ -61,15 +61,19 @@ const char* gsdr_strerror(int32_t status) {
 const char* gsdr_last_error(void) { return g_last_error.c_str(); }
 
 // ----------------------------------------------------------------- records
-int gsdr_records_read_fasta(const char* path, gsdr_records** out) {
+static int read_records(const char* path, bool length_order, gsdr_records** out) {
   if (!path || !out) return fail(GSDR_E_INVALID, "null argument");
   auto r = std::make_unique<gsdr_records>();
   std::string err;
-  int st = read_fasta(path, r->r, err);
+  int st = read_fasta(path, r->r, err, host_threads(), length_order);
   if (st != GSDR_OK) return fail(st, err);
   *out = r.release();
   return GSDR_OK;
 }
+
+int gsdr_records_read_fasta(const char* path, gsdr_records** out) { return read_records(path, false, out); }
+
+int gsdr_records_read_fasta_sorted(const char* path, gsdr_records** out) { return read_records(path, true, out); }
 
 int gsdr_records_from_buffers(const char* names, const int64_t* name_off, const char* data,
                               const int64_t* data_off, int64_t count, gsdr_records** out) {
@@ -110,9 +114,17 @@ int gsdr_records_write_fasta(const gsdr_records* r, const char* path, int32_t li
 
 int gsdr_records_sort(gsdr_records* r, int64_t* perm_out) {
   if (!r) return fail(GSDR_E_INVALID, "null records");
-  std::vector<int64_t> perm = length_order(r->r);
-  r->r = permute(r->r, perm, host_threads());
-  if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
+  Records& rec = r->r;
+  if (!rec.length_ordered) {
+    std::vector<int64_t> perm = length_order(rec);
+    std::vector<int64_t> prior = std::move(rec.order);
+    rec = permute(rec, perm, host_threads());
+    if (!prior.empty())
+      for (auto& p : perm) p = prior[(size_t)p];
+    rec.length_ordered = true;
+    rec.order = std::move(perm);
+  }
+  if (perm_out) std::copy(rec.order.begin(), rec.order.end(), perm_out);
   return GSDR_OK;
 }
 

@@ -136,7 +136,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   const StepLaunch L = make_step_launch(c, njobs, smax, max_k, w.jobs_left);
 
   double ms_filter = 0, ms_align = 0;
-  int64_t launches = 2, replays = 0, steps_launched = 0;
+  int64_t launches = 2, steps_launched = 0;
   // a job needs at most n steps; every step fixes at least one representative
   const int64_t step_limit = max_n + 2;
   if (timing) {
@@ -185,7 +185,6 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       if (e == hipSuccess) e = hipMemcpyAsync(h_left, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q);
       if (e == hipSuccess) e = hipStreamSynchronize(q);
       if (e != hipSuccess) { rc = hip_fail(e, "graph replay"); break; }
-      ++replays;
       steps_launched += B;
       launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1);
       if (*h_left == 0) break;

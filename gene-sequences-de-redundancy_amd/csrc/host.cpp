@@ -6,7 +6,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <new>
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -16,6 +18,24 @@
 #include "status.hpp"
 
 namespace gsdr {
+
+// ------------------------------------------------------------------ memory
+static constexpr size_t kHuge = size_t(2) << 20;
+
+void* big_alloc(size_t bytes) {
+  if (bytes < 16 * kHuge) {
+    void* p = std::malloc(bytes ? bytes : 1);
+    if (!p) throw std::bad_alloc();
+    return p;
+  }
+  const size_t len = (bytes + kHuge - 1) & ~(kHuge - 1);
+  void* p = std::aligned_alloc(kHuge, len);
+  if (!p) throw std::bad_alloc();
+  madvise(p, len, MADV_HUGEPAGE);
+  return p;
+}
+
+void big_free(void* p, size_t) noexcept { std::free(p); }
 
 // ---------------------------------------------------------------- encoding
 static uint8_t code_of(unsigned char c) {
@@ -34,6 +54,73 @@ struct CodeTable {
 static const CodeTable kTable;
 static inline uint8_t base_code(unsigned char c) { return kTable.v[c]; }
 
+// ------------------------------------------------------------------- order
+// func.cpp:105-107: std::sort(reads, [](a, b) { return a.data.size() >
+// b.data.size(); }). libstdc++'s introsort makes the same moves for any
+// element type given the same comparison results, so sorting (length, index)
+// keys with the same comparator yields the reference's permutation,
+// including its (unstable) order of equal lengths.
+namespace {
+template <class I>
+std::vector<int64_t> sort_keys(const std::vector<int64_t>& lens) {
+  struct Key { I len; I idx; };
+  const int64_t n = (int64_t)lens.size();
+  std::vector<Key> keys((size_t)n);
+  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{(I)lens[(size_t)i], (I)i};
+  std::sort(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; });
+  std::vector<int64_t> perm((size_t)n);
+  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int64_t)keys[(size_t)i].idx;
+  return perm;
+}
+
+// The moves depend only on the comparison results, so 32-bit keys (when
+// lengths and indices fit) give the same permutation with half the traffic.
+std::vector<int64_t> order_by_length(const std::vector<int64_t>& lens) {
+  const int64_t n = (int64_t)lens.size();
+  bool narrow = n <= INT32_MAX;
+  for (int64_t i = 0; narrow && i < n; ++i) narrow = lens[(size_t)i] <= INT32_MAX;
+  return narrow ? sort_keys<int32_t>(lens) : sort_keys<int64_t>(lens);
+}
+
+// Runs body(lo, hi) over [0, n) in chunks handed out dynamically.
+template <class F>
+void parallel_chunks(int64_t n, int64_t chunk, int threads, F body) {
+  if (threads <= 1 || n <= chunk) {
+    if (n > 0) body((int64_t)0, n);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto run = [&]() {
+    for (int64_t lo; (lo = next.fetch_add(chunk)) < n;) body(lo, std::min(n, lo + chunk));
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < threads; ++t) th.emplace_back(run);
+  run();
+  for (auto& x : th) x.join();
+}
+
+// Copies the bytes of [p, e) other than '\n' to dst; returns the bytes written.
+int64_t copy_lines(const char* p, const char* e, char* dst) {
+  char* d = dst;
+  while (p < e) {
+    const char* nl = (const char*)memchr(p, '\n', (size_t)(e - p));
+    const char* q = nl ? nl : e;
+    std::memcpy(d, p, (size_t)(q - p));
+    d += q - p;
+    p = q + 1;
+  }
+  return d - dst;
+}
+
+}  // namespace
+
+std::vector<int64_t> length_order(const Records& r) {
+  const int64_t n = r.count();
+  std::vector<int64_t> lens((size_t)n);
+  for (int64_t i = 0; i < n; ++i) lens[(size_t)i] = r.len(i);
+  return order_by_length(lens);
+}
+
 // ------------------------------------------------------------------ reader
 // func.cpp:70-104. The first line of the file is the first record's name
 // (verbatim). Each later line that starts with '>' starts a new record;
@@ -42,60 +129,137 @@ static inline uint8_t base_code(unsigned char c) { return kTable.v[c]; }
 // reference loops forever (no trailing newline, a header as the last line,
 // an empty file) this reader ends: the last partial line counts as a line
 // and a trailing header becomes a record with empty data.
-int read_fasta(const char* path, Records& out, std::string& err) {
+//
+// A record therefore starts at offset 0 and at every '>' that follows a
+// '\n'; its name runs to the first '\n' after the start and its data is
+// every byte up to the next start except '\n'. The reader finds the starts
+// in parallel slices of the file, measures each record, and copies the
+// records (in file or length order) in parallel.
+int read_fasta(const char* path, Records& out, std::string& err, int threads, bool length_order) {
   out.clear();
   int fd = ::open(path, O_RDONLY);
   if (fd < 0) { err = std::string("cannot open input file: ") + path; return GSDR_E_IO; }
   struct stat st;
   if (fstat(fd, &st) != 0) { ::close(fd); err = "cannot stat input file"; return GSDR_E_IO; }
-  size_t size = (size_t)st.st_size;
+  const int64_t size = (int64_t)st.st_size;
   if (size == 0) { ::close(fd); err = "input file is empty"; return GSDR_E_INPUT; }
-  const char* buf = (const char*)mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+  const char* buf = (const char*)mmap(nullptr, (size_t)size, PROT_READ, MAP_PRIVATE, fd, 0);
   ::close(fd);
   if (buf == MAP_FAILED) { err = "cannot map input file"; return GSDR_E_IO; }
-  madvise((void*)buf, size, MADV_SEQUENTIAL);
+  if (threads < 1) threads = 1;
 
-  out.names.reserve(size / 64 + 16);
-  out.data.reserve(size);
-  size_t pos = 0;
-  bool first = true;
-  while (pos < size) {
-    const char* nl = (const char*)memchr(buf + pos, '\n', size - pos);
-    size_t end = nl ? (size_t)(nl - buf) : size;
-    const char* line = buf + pos;
-    size_t len = end - pos;
-    if (first || (len > 0 && line[0] == '>')) {
-      if (!first) out.data_off.push_back((int64_t)out.data.size());
-      out.names.insert(out.names.end(), line, line + len);
-      out.name_off.push_back((int64_t)out.names.size());
-      first = false;
-    } else {
-      out.data.insert(out.data.end(), line, line + len);
+  // One pass over slices of the file finds the record starts and counts
+  // the '\n' bytes of each record. Slice sl scans the '\n' bytes at
+  // x in [a - 1, b - 1) (the last slice up to the end of the file): each
+  // belongs to the record holding x, and one followed by '>' starts a record
+  // at x + 1. `head` counts the ones that belong to a record of an earlier
+  // slice.
+  struct Slice {
+    std::vector<int64_t> start, hdr, nl;
+    int64_t head = 0;
+  };
+  const int64_t nslice = std::max<int64_t>(1, std::min<int64_t>(threads * 4, size >> 20));
+  std::vector<Slice> part((size_t)nslice);
+  parallel_chunks(nslice, 1, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t sl = lo; sl < hi; ++sl) {
+      const int64_t a = size * sl / nslice, b = size * (sl + 1) / nslice;
+      Slice& v = part[(size_t)sl];
+      int64_t* cnt = &v.head;
+      bool want_hdr = false;
+      if (a == 0) {
+        v.start.push_back(0); v.hdr.push_back(-1); v.nl.push_back(0);
+        cnt = &v.nl.back();
+        want_hdr = true;
+      }
+      const char* p = buf + std::max<int64_t>(a - 1, 0);
+      const char* e = buf + (sl + 1 == nslice ? size : b - 1);
+      while (p < e) {
+        const char* nl = (const char*)memchr(p, '\n', (size_t)(e - p));
+        if (!nl) break;
+        ++*cnt;
+        if (want_hdr) { v.hdr.back() = nl - buf; want_hdr = false; }
+        if (nl + 1 < buf + size && nl[1] == '>') {
+          v.start.push_back(nl + 1 - buf); v.hdr.push_back(-1); v.nl.push_back(0);
+          cnt = &v.nl.back();
+          want_hdr = true;
+        }
+        p = nl + 1;
+      }
     }
-    pos = end + 1;
-  }
-  out.data_off.push_back((int64_t)out.data.size());
-  munmap((void*)buf, size);
-  return GSDR_OK;
-}
+  });
+  std::vector<int64_t> first((size_t)nslice + 1, 0);
+  for (int64_t sl = 0; sl < nslice; ++sl) first[(size_t)sl + 1] = first[(size_t)sl] + (int64_t)part[(size_t)sl].start.size();
+  const int64_t n = first[(size_t)nslice];
+  std::vector<int64_t> start((size_t)n + 1), hdr((size_t)n), dlen((size_t)n);
+  parallel_chunks(nslice, 1, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t sl = lo; sl < hi; ++sl) {
+      Slice& v = part[(size_t)sl];
+      const size_t r0 = (size_t)first[(size_t)sl];
+      std::copy(v.start.begin(), v.start.end(), start.begin() + r0);
+      std::copy(v.hdr.begin(), v.hdr.end(), hdr.begin() + r0);
+      std::copy(v.nl.begin(), v.nl.end(), dlen.begin() + r0);
+    }
+  });
+  // '\n' bytes at the head of a slice belong to the last record before it
+  for (int64_t sl = 1; sl < nslice; ++sl)
+    if (first[(size_t)sl] > 0) dlen[(size_t)first[(size_t)sl] - 1] += part[(size_t)sl].head;
+  part.clear();
+  start[(size_t)n] = size;
+  // bytes of [s, e) = name + every '\n' + data; a name without '\n' ends
+  // at e (the last record); a name cut by a slice edge is found here
+  parallel_chunks(n, 4096, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r) {
+      const int64_t s = start[(size_t)r], e = start[(size_t)r + 1];
+      int64_t h = hdr[(size_t)r];
+      if (h < 0) {
+        const char* nl = (const char*)memchr(buf + s, '\n', (size_t)(e - s));
+        h = nl ? nl - buf : e;
+        hdr[(size_t)r] = h;
+      }
+      dlen[(size_t)r] = (e - s) - (h - s) - dlen[(size_t)r];
+    }
+  });
 
-// ------------------------------------------------------------------- order
-// func.cpp:105-107: std::sort(reads, [](a, b) { return a.data.size() >
-// b.data.size(); }). libstdc++'s introsort makes the same moves for any
-// element type given the same comparison results, so sorting (length, index)
-// keys with the same comparator yields the reference's permutation,
-// including its (unstable) order of equal lengths.
-namespace {
-struct Key { int64_t len; int64_t idx; };
-}
-std::vector<int64_t> length_order(const Records& r) {
-  const int64_t n = r.count();
-  std::vector<Key> keys((size_t)n);
-  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{r.len(i), i};
-  std::sort(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; });
-  std::vector<int64_t> perm((size_t)n);
-  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = keys[(size_t)i].idx;
-  return perm;
+  std::vector<int64_t> perm;
+  if (length_order) perm = order_by_length(dlen);
+  // records are copied in file order (sequential reads of the file) to
+  // their place in the output order
+  std::vector<int64_t> slot;
+  if (length_order) {
+    slot.resize((size_t)n);
+    parallel_chunks(n, 65536, threads, [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) slot[(size_t)perm[(size_t)i]] = i;
+    });
+  }
+  out.name_off.assign((size_t)n + 1, 0);
+  out.data_off.assign((size_t)n + 1, 0);
+  parallel_chunks(n, 8192, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r) {
+      const int64_t i = length_order ? slot[(size_t)r] : r;
+      out.name_off[(size_t)i + 1] = hdr[(size_t)r] - start[(size_t)r];
+      out.data_off[(size_t)i + 1] = dlen[(size_t)r];
+    }
+  });
+  for (int64_t i = 0; i < n; ++i) {
+    out.name_off[(size_t)i + 1] += out.name_off[(size_t)i];
+    out.data_off[(size_t)i + 1] += out.data_off[(size_t)i];
+  }
+  out.names.resize((size_t)out.name_off[(size_t)n]);
+  out.data.resize((size_t)out.data_off[(size_t)n]);
+  parallel_chunks(n, 1024, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r) {
+      const int64_t i = length_order ? slot[(size_t)r] : r;
+      const int64_t s = start[(size_t)r], e = start[(size_t)r + 1], h = hdr[(size_t)r];
+      std::memcpy(out.names.data() + out.name_off[(size_t)i], buf + s, (size_t)(h - s));
+      if (h + 1 < e) copy_lines(buf + h + 1, buf + e, out.data.data() + out.data_off[(size_t)i]);
+    }
+  });
+  munmap((void*)buf, (size_t)size);
+  if (length_order) {
+    out.length_ordered = true;
+    out.order = std::move(perm);
+  }
+  return GSDR_OK;
 }
 
 Records permute(const Records& r, const std::vector<int64_t>& perm, int threads) {

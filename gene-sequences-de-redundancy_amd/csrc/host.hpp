@@ -6,22 +6,50 @@
 // same shared library as the HIP kernels.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace gsdr {
 
 // One FASTA record as the reference stores it (func.h:17-20): the header line
 // verbatim (including '>') and the raw concatenation of the following lines.
+// Large host buffers: 2 MiB aligned and marked for huge pages, so that
+// filling them costs few page faults.
+void* big_alloc(size_t bytes);
+void big_free(void* p, size_t bytes) noexcept;
+
+// Byte buffer whose resize leaves new bytes uninitialised, so that large
+// buffers are first touched by the threads that fill them.
+template <class T>
+struct NoInitAlloc {
+  using value_type = T;
+  NoInitAlloc() = default;
+  template <class U> NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  T* allocate(size_t n) { return (T*)big_alloc(n * sizeof(T)); }
+  void deallocate(T* p, size_t n) noexcept { big_free(p, n * sizeof(T)); }
+  template <class U> void construct(U* p) noexcept { ::new ((void*)p) U; }
+  template <class U, class... A> void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+  template <class U> bool operator==(const NoInitAlloc<U>&) const noexcept { return true; }
+  template <class U> bool operator!=(const NoInitAlloc<U>&) const noexcept { return false; }
+};
+using Bytes = std::vector<char, NoInitAlloc<char>>;
+
 struct Records {
-  std::vector<char> names;          // concatenated header bytes
+  Bytes names;                      // concatenated header bytes
   std::vector<int64_t> name_off;    // size n+1
-  std::vector<char> data;           // concatenated raw sequence bytes
+  Bytes data;                       // concatenated raw sequence bytes
   std::vector<int64_t> data_off;    // size n+1
+  // Set once the records are in the reference's length order; `order` then
+  // holds the input position of each record.
+  bool length_ordered = false;
+  std::vector<int64_t> order;
   int64_t count() const { return (int64_t)data_off.size() - 1; }
   int64_t len(int64_t i) const { return data_off[i + 1] - data_off[i]; }
   void clear() {
     names.clear(); name_off.assign(1, 0); data.clear(); data_off.assign(1, 0);
+    length_ordered = false; order.clear();
   }
   void push(const char* nm, int64_t nl, const char* d, int64_t dl) {
     names.insert(names.end(), nm, nm + nl); name_off.push_back((int64_t)names.size());
@@ -30,9 +58,12 @@ struct Records {
   Records() { clear(); }
 };
 
-// Reads a FASTA file with the reference's record semantics (func.cpp:70-104).
-// Returns 0 on success, a gsdr status code otherwise; `err` gets a message.
-int read_fasta(const char* path, Records& out, std::string& err);
+// Reads a FASTA file with the reference's record semantics (func.cpp:70-104)
+// on `threads` threads. With `length_order` the records come out in the
+// reference's length order (func.cpp:105-107) with `order` set, in the same
+// pass that copies them. Returns 0 on success, a gsdr status code otherwise;
+// `err` gets a message.
+int read_fasta(const char* path, Records& out, std::string& err, int threads = 1, bool length_order = false);
 
 // Stable-free length order of the reference (func.cpp:105-107): libstdc++
 // std::sort with "longer first". Returns perm with perm[sorted] = original.

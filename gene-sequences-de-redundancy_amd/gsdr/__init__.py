@@ -71,6 +71,7 @@ _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 SIGNATURES = [
     ("gsdr_records_read_fasta", C.c_int, [C.c_char_p, _PP]),
+    ("gsdr_records_read_fasta_sorted", C.c_int, [C.c_char_p, _PP]),
     ("gsdr_records_from_buffers", C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.c_char_p,
                                             C.POINTER(C.c_int64), C.c_int64, _PP]),
     ("gsdr_records_generate", C.c_int, [C.c_int32, C.c_uint64, C.c_int64, C.c_double, _PP]),
@@ -150,9 +151,12 @@ class Records:
         self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
 
     @staticmethod
-    def read_fasta(path):
+    def read_fasta(path, length_order=False):
+        """File order, or with length_order the reference's length order
+        (what sort() would give); sort() then only reports the permutation."""
         h = C.c_void_p()
-        _check(lib().gsdr_records_read_fasta(os.fsencode(path), C.byref(h)), "read_fasta")
+        fn = lib().gsdr_records_read_fasta_sorted if length_order else lib().gsdr_records_read_fasta
+        _check(fn(os.fsencode(path), C.byref(h)), "read_fasta")
         return Records(h)
 
     @staticmethod

@@ -84,8 +84,13 @@ typedef struct gsdr_stats {
 /* ---- records (host side): replaces readFile/saveFile (func.cpp:70-112,
  *      785-799) and the Read vector (func.h:17-20) ---- */
 
-/* Reads FASTA with the reference's record split (func.cpp:70-104). */
+/* Reads FASTA with the reference's record split (func.cpp:70-104), in file
+ * order. */
 int gsdr_records_read_fasta(const char *path, gsdr_records **out);
+/* Reads FASTA (func.cpp:70-104) straight into the reference's length order
+ * (func.cpp:105-107): the same records and order as read_fasta followed by
+ * sort, with one copy of the data instead of two. */
+int gsdr_records_read_fasta_sorted(const char *path, gsdr_records **out);
 /* Builds records from caller buffers (offsets have count+1 entries). */
 int gsdr_records_from_buffers(const char *names, const int64_t *name_off, const char *data,
                               const int64_t *data_off, int64_t count, gsdr_records **out);
@@ -100,7 +105,9 @@ int gsdr_records_generate(int32_t law, uint64_t seed, int64_t n, double median,
                           gsdr_records **out);
 int gsdr_records_write_fasta(const gsdr_records *r, const char *path, int32_t line_width);
 /* Reorders in place into the reference's length order (func.cpp:105-107);
- * perm_out (may be NULL, count entries) gets the original index per slot. */
+ * perm_out (may be NULL, count entries) gets the original index per slot.
+ * Records already in that order (sorted, or read with read_fasta_sorted)
+ * stay as they are and perm_out gets the same permutation again. */
 int gsdr_records_sort(gsdr_records *r, int64_t *perm_out);
 int64_t gsdr_records_count(const gsdr_records *r);
 int64_t gsdr_records_bases(const gsdr_records *r);

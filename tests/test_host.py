@@ -86,6 +86,39 @@ def test_fasta_roundtrip_and_sort(tmp_path):
     assert np.array_equal(perm, O.sort_order(lens))
 
 
+def test_parallel_reader_awkward_file(tmp_path):
+    """A file of several MiB read in parallel slices: ragged line widths,
+    blank lines, CRLF, '>' inside data lines, a header right after a header,
+    data before the first header line and no trailing newline; both readers
+    against the oracle's reader and order."""
+    rng = np.random.default_rng(11)
+    parts = [b"first line is a name\n", b"ACGT\n"]
+    alphabet = np.frombuffer(b"ACGTNacgt>", dtype=np.uint8)
+    while sum(len(x) for x in parts) < (6 << 20):
+        parts.append(b">r%d\n" % len(parts))
+        kind = rng.integers(0, 8)
+        if kind == 0:
+            continue  # header followed by a header
+        for _ in range(int(rng.integers(1, 12))):
+            w = int(rng.integers(0, 200))
+            line = alphabet[rng.integers(0, len(alphabet) - 1 if kind < 6 else len(alphabet), w)].tobytes()
+            parts.append(line + (b"\r\n" if kind == 7 else b"\n"))
+    parts.append(b">tail\nACGTACGT")
+    p = tmp_path / "awkward.fasta"
+    p.write_bytes(b"".join(parts))
+    expect = O.read_fasta(str(p))
+    assert len(expect) > 10000
+    r = gsdr.Records.read_fasta(str(p))
+    assert len(r) == len(expect)
+    assert all(r.get(i) == expect[i] for i in range(len(expect)))
+    order = O.sort_order([len(d) for _, d in expect])
+    s = gsdr.Records.read_fasta(str(p), length_order=True)
+    assert all(s.get(i) == expect[order[i]] for i in range(len(expect)))
+    assert np.array_equal(s.sort(), order)
+    assert np.array_equal(r.sort(), order)
+    assert np.array_equal(r.sort(), order)  # already in order: same permutation again
+
+
 def test_result_and_membership_writers(tmp_path):
     r = gsdr.Records.from_lists([b"ACGTAC", b"ACGA", b"AC"], names=[b">a", b">b", b">c"])
     cl = np.array([0, 0, 2], dtype=np.int32)

@@ -236,6 +236,11 @@ def test_fasta_edge_cases(tmp_path, name):
     assert O.read_fasta(str(p)) == EDGE_EXPECT[name]
     r = gsdr.Records.read_fasta(str(p))
     assert [r.get(i) for i in range(len(r))] == EDGE_EXPECT[name]
+    expect = EDGE_EXPECT[name]
+    order = O.sort_order([len(d) for _, d in expect])
+    s = gsdr.Records.read_fasta(str(p), length_order=True)
+    assert [s.get(i) for i in range(len(s))] == [expect[i] for i in order]
+    assert np.array_equal(s.sort(), order)
 
 
 # ---------------------------------------------------------- greedy driver
