@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.0, help="override the workload's threshold")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
+    ap.add_argument("--tune", action="store_true", help="autotune the launch knobs on the workload first")
+    ap.add_argument("--tune-trials", type=int, default=12)
     ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
     ap.add_argument("--jobs-count", type=int, default=512)
     ap.add_argument("--jobs-scale", type=float, default=1.0, help="scale the upper job size")
@@ -256,15 +258,23 @@ def main():
     mode = gsdr.MODE_BATCHED if args.mode == "batched" else gsdr.MODE_PLAIN
     print(f"# {desc}; n={len(recs)} bases={recs.bases} k={k} store={store.footprint / 2**20:.1f} MiB "
           f"device={eng.info()['name']}", file=sys.stderr, flush=True)
+    tuning = None
+    if args.tune:
+        # untimed: the autotuner clusters the store under several launch
+        # settings (all give the same clusters) and keeps the fastest
+        tu = store.autotune(t, k, max_trials=args.tune_trials)
+        tuning = tu.as_dict()
+        print(f"# autotune: {tuning}", file=sys.stderr, flush=True)
+    tune = tuning is not None
     ref = None
     for i in range(args.warmup):
-        cl, st = store.cluster(t, k, mode=mode)
+        cl, st = store.cluster(t, k, mode=mode, tune=tune)
         ref = cl if ref is None else ref
         print(f"# warmup {i}: {st.ms_total:.1f} ms, reps {st.reps}, rounds {st.rounds}", file=sys.stderr, flush=True)
     times, stats = [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        cl, st = store.cluster(t, k, mode=mode)
+        cl, st = store.cluster(t, k, mode=mode, tune=tune)
         times.append(time.perf_counter() - t0)
         stats.append(st)
         if ref is not None:
@@ -275,7 +285,7 @@ def main():
     ms = 1e3 * sum(times) / len(times)
     # one more step with HIP events around every filter and alignment launch
     # (on the engine's stream) for the kernel-level figures
-    clt, st = store.cluster(t, k, mode=mode, timing=True)
+    clt, st = store.cluster(t, k, mode=mode, timing=True, tune=tune)
     assert (clt == ref).all(), "timed step differs"
     if args.dump:
         ref.astype("<i4").tofile(args.dump)
@@ -298,7 +308,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded generator of the stated law)",
         "config": {"workload": desc, "records": len(recs), "bases": bases, "threshold": t, "word_length": k,
-                   "form": args.mode},
+                   "form": args.mode, "tuned": tuning},
         "banded_gcups": round(st.cells_band / align_s / 1e9, 3) if align_s > 0 else None,
         "reps": int(st.reps),
         "rounds": int(st.rounds),

@@ -257,6 +257,43 @@ int gsdr_cluster_jobs(gsdr_ctx* c, int32_t njobs, gsdr_store* const* stores, con
   return cluster_jobs(c, njobs, stores, params, cluster_out, stats);
 }
 
+int gsdr_autotune(gsdr_ctx* c, gsdr_store* s, const gsdr_params* p, int32_t max_trials, gsdr_tuning* out) {
+  if (!c || !s || !p) return fail(GSDR_E_INVALID, "null argument");
+  if (s->ctx != c) return fail(GSDR_E_INVALID, "store belongs to another context");
+  return autotune(c, *s, *p, max_trials, out);
+}
+
+int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
+  if (!c) return fail(GSDR_E_INVALID, "null context");
+  if (!t) {
+    c->tuned = Tuning();
+    c->has_tuning = false;
+    return GSDR_OK;
+  }
+  if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || t->filter_blocks < 1 ||
+      t->filter_blocks > 64 || t->align_blocks < 1 || t->align_blocks > 64)
+    return fail(GSDR_E_INVALID, "tuning out of range");
+  Tuning T;
+  T.swap_ratio = t->swap_ratio;
+  T.spec_max = t->spec_max;
+  T.filter_blocks = t->filter_blocks;
+  T.align_blocks = t->align_blocks;
+  c->tuned = T;
+  c->has_tuning = true;
+  return GSDR_OK;
+}
+
+int gsdr_get_tuning(const gsdr_ctx* c, gsdr_tuning* out) {
+  if (!c || !out) return fail(GSDR_E_INVALID, "null argument");
+  const Tuning T = tuning_for(c, true);
+  std::memset(out, 0, sizeof(*out));
+  out->swap_ratio = T.swap_ratio;
+  out->spec_max = T.spec_max;
+  out->filter_blocks = T.filter_blocks;
+  out->align_blocks = T.align_blocks;
+  return GSDR_OK;
+}
+
 int gsdr_align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int32_t nseq,
                      const int32_t* pairs, const int32_t* bc, int32_t npairs, int32_t shape,
                      int32_t* accept_out, double* cells_exec_out) {

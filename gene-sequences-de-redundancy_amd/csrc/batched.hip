@@ -40,15 +40,15 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   return launch_resolve(L.ra, q);
 }
 
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left) {
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left, const Tuning& T) {
   Workspace& w = c->ws;
   StepLaunch L;
   L.max_k = max_k;
   L.smax = smax;
-  L.filter_grid = c->cu_count * 8;
+  L.filter_grid = c->cu_count * T.filter_blocks;
   L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
   for (int sh = 0; sh < kNumShapes; ++sh) {
-    L.align_grid[sh] = c->cu_count * 4;
+    L.align_grid[sh] = c->cu_count * T.align_blocks;
     // phase A holds at most k_spec^2/2 pairs per job
     const int segs = 64 / kShapes[sh].s;
     L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));
@@ -56,6 +56,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t
   L.fa_main.jobs = w.jobs;
   L.fa_main.njobs = njobs;
   L.fa_main.min_shape = 0;
+  L.fa_main.swap_ratio = T.swap_ratio;
   L.fa_main.phase = PHASE_MAIN;
   L.fa_main.items = w.items;
   L.fa_main.item_cap = w.item_cap;
@@ -94,7 +95,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   if (njobs > kMaxJobsPerLaunch) return fail(GSDR_E_LIMIT, "more than 1024 jobs in one call");
   HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
   int st;
-  int max_k = 4, smax = 0, timing = 0, spec = 0;
+  int max_k = 4, smax = 0, timing = 0, spec = 0, tune = 0;
   int64_t total_n = 0, total_bases = 0, max_n = 0;
   std::vector<JobDev> jobs((size_t)njobs);
   for (int j = 0; j < njobs; ++j) {
@@ -111,11 +112,13 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     max_n = std::max<int64_t>(max_n, s.n);
     timing |= params[j].timing;
     spec = std::max(spec, params[j].spec_reps);
+    tune |= params[j].tune;
   }
+  const Tuning T = tuning_for(c, tune != 0);
   for (int j = 0; j + 1 < njobs; ++j)
     for (int i = j + 1; i < njobs; ++i)
       if (stores[j] == stores[i]) return fail(GSDR_E_INVALID, "the same store appears twice");
-  const int k_max = spec > 0 ? std::min(spec, kMaxSpec) : kMaxSpec;
+  const int k_max = std::min(spec > 0 ? std::min(spec, kMaxSpec) : kMaxSpec, T.spec_max);
   // alignment list capacity per shape: enough for several speculative
   // representatives over every job's candidates
   const int64_t item_cap = std::min<int64_t>(std::max<int64_t>(8 * total_n, max_n), 32ll << 20);
@@ -133,7 +136,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs.data(), njobs, k_max, w.jobs_left, q), "init");
 
-  const StepLaunch L = make_step_launch(c, njobs, smax, max_k, w.jobs_left);
+  const StepLaunch L = make_step_launch(c, njobs, smax, max_k, w.jobs_left, T);
 
   double ms_filter = 0, ms_align = 0;
   int64_t launches = 2, steps_launched = 0;

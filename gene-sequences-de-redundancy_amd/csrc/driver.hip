@@ -199,7 +199,8 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   int64_t filter_launches = 0, align_launches = 0, rounds = 0;
   int nc = n, first = 0;
   bool done = n == 0;
-  const int cap_blocks = c->cu_count * 4;
+  const Tuning T = tuning_for(c, p.tune != 0);
+  const int cap_blocks = c->cu_count * T.align_blocks;
   while (!done) {
     ++rounds;
     const int units = (nc + kFilterChunk - 1) / kFilterChunk;
@@ -208,12 +209,13 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     fa.jobs = w.jobs;
     fa.njobs = 1;
     fa.min_shape = 0;
+    fa.swap_ratio = T.swap_ratio;
     fa.phase = PHASE_MAIN;
     fa.items = w.items;
     fa.item_cap = w.item_cap;
     fa.bin_count = w.counters;
     fa.stats = w.stats;
-    HIP_TRY(launch_filter(fa, k, std::max(1, std::min(units, cap_blocks * 2)), q), "filter launch");
+    HIP_TRY(launch_filter(fa, k, std::max(1, std::min(units, c->cu_count * T.filter_blocks)), q), "filter launch");
     ++launches;
     ++filter_launches;
     if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");

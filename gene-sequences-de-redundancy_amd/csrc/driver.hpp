@@ -28,8 +28,20 @@ struct Workspace {
   int64_t bytes = 0;
 };
 
+// Launch knobs that change speed, never results: the role-swap ratio, the
+// cap on speculative representatives per step and the filter / align
+// workgroups per CU. gsdr_autotune measures them on a store.
+struct Tuning {
+  int swap_ratio = gsdr::kSwapRatio;
+  int spec_max = gsdr::kMaxSpec;
+  int filter_blocks = 8;
+  int align_blocks = 4;
+};
+
 struct gsdr_ctx {
   int device = 0;
+  Tuning tuned;                       // used by calls with params.tune = 1
+  bool has_tuning = false;
   hipStream_t stream = nullptr;       // kernels
   hipStream_t upload = nullptr;       // store uploads (may overlap kernels of another job)
   int cu_count = 0;
@@ -138,6 +150,10 @@ struct StepLaunch {
   int reps_grid[kNumShapes] = {};
   int reps_filter_grid = 1;
 };
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left);
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left,
+                            const Tuning& T = Tuning());
+// The knobs a call runs with: the context's tuned ones when asked for and present.
+inline Tuning tuning_for(const gsdr_ctx* c, bool want) { return want && c->has_tuning ? c->tuned : Tuning(); }
+int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int max_trials, gsdr_tuning* out);
 hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev);
 }  // namespace gsdr

@@ -154,10 +154,10 @@ struct AlignItem {
   int32_t text, query, job, slot;
 };
 constexpr int kSwapFlag = 256;
-constexpr int kSwapRatio = 4;
+constexpr int kSwapRatio = 4;  // default; a tuned context may choose another (0 = never)
 
-__host__ __device__ inline bool swap_roles(int n_rep, int m_cand) {
-  return n_rep >= kSwapRatio * m_cand;
+__host__ __device__ inline bool swap_roles(int n_rep, int m_cand, int ratio = kSwapRatio) {
+  return ratio > 0 && (long long)n_rep >= (long long)ratio * m_cand;
 }
 
 }  // namespace gsdr

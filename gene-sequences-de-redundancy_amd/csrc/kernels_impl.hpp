@@ -10,6 +10,7 @@ struct FilterArgs {
   const JobDev* jobs;
   int32_t njobs;
   int32_t min_shape;
+  int32_t swap_ratio;       // role swap when n_rep >= swap_ratio * m_cand (0 = never)
   int32_t phase;            // FilterPhase
   AlignItem* items;         // per-shape lists: items + shape * item_cap
   int64_t item_cap;
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       bytes += 16;
       cband += (unsigned long long)band_cells(n, mq, c);
       cfull += (unsigned long long)n * (unsigned long long)mq;
-      const bool sw = swap_roles(n, mq);
+      const bool sw = swap_roles(n, mq, a.swap_ratio);
       const int sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
       const int at = atomicAdd(&a.bin_count[sh], 1);
       AlignItem it;

@@ -1,0 +1,113 @@
+// Autotuner for the launch knobs of the batched form: role-swap ratio, cap
+// on speculative representatives per step, filter and alignment workgroups
+// per CU. None of them changes a result (every setting computes the same
+// greedy clustering), so the tuner times whole clusterings of the caller's
+// store and keeps the fastest setting; it also checks that every trial
+// returned the same clusters. No counterpart in the reference, whose launch
+// shapes are fixed (func.cpp:695-781).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "driver.hpp"
+
+namespace gsdr {
+
+namespace {
+
+struct Knob {
+  int Tuning::*field;
+  std::vector<int> values;
+};
+
+}  // namespace
+
+int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, gsdr_tuning* out) {
+  if (max_trials <= 0) max_trials = 16;
+  gsdr_params p = p0;
+  p.mode = GSDR_MODE_BATCHED;
+  p.timing = 0;
+  p.tune = 1;
+  const Tuning saved = c->tuned;
+  const bool had = c->has_tuning;
+  const size_t n = (size_t)s.n;
+  std::vector<int32_t> ref(n), got(n);
+  // one clustering under knobs T; its clusters must equal the first run's
+  auto run = [&](const Tuning& T, double* ms, bool first) -> int {
+    c->tuned = T;
+    c->has_tuning = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    int st = cluster_store(c, s, p, first ? ref.data() : got.data(), nullptr);
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) return st;
+    if (!first && std::memcmp(ref.data(), got.data(), n * sizeof(int32_t)) != 0)
+      return fail(GSDR_E_DEVICE, "autotune: clusters differ between launch settings");
+    return GSDR_OK;
+  };
+  auto finish = [&](int st) {
+    if (st) {
+      c->tuned = saved;
+      c->has_tuning = had;
+    }
+    return st;
+  };
+  Tuning best;
+  double ms = 0, ms_default = 0;
+  int st = run(best, &ms, true);  // warm-up: workspace, graph, caches
+  if (st) return finish(st);
+  if ((st = run(best, &ms_default, false))) return finish(st);
+  double ms_best = ms_default;
+  int trials = 1;
+  const Knob knobs[] = {
+      {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
+      {&Tuning::spec_max, {16, 32}},
+      {&Tuning::align_blocks, {2, 8}},
+      {&Tuning::filter_blocks, {4, 16}},
+  };
+  for (const Knob& k : knobs) {
+    for (int v : k.values) {
+      if (trials >= max_trials) break;
+      Tuning T = best;
+      if (T.*k.field == v) continue;
+      T.*k.field = v;
+      if ((st = run(T, &ms, false))) return finish(st);
+      ++trials;
+      // keep a change only when it is clearly faster than the run-to-run noise
+      if (ms < 0.98 * ms_best) {
+        best = T;
+        ms_best = ms;
+      }
+    }
+  }
+  // confirm a changed setting once more against the defaults
+  if (trials < max_trials &&
+      (best.swap_ratio != Tuning().swap_ratio || best.spec_max != Tuning().spec_max ||
+       best.align_blocks != Tuning().align_blocks || best.filter_blocks != Tuning().filter_blocks)) {
+    double again = 0;
+    if ((st = run(best, &again, false))) return finish(st);
+    ++trials;
+    if (again >= ms_default) {
+      best = Tuning();
+      ms_best = ms_default;
+    } else {
+      ms_best = std::min(ms_best, again);
+    }
+  }
+  c->tuned = best;
+  c->has_tuning = true;
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->swap_ratio = best.swap_ratio;
+    out->spec_max = best.spec_max;
+    out->filter_blocks = best.filter_blocks;
+    out->align_blocks = best.align_blocks;
+    out->trials = trials;
+    out->ms_default = ms_default;
+    out->ms_tuned = ms_best;
+  }
+  return GSDR_OK;
+}
+
+}  // namespace gsdr

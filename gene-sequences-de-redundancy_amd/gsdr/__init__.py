@@ -66,6 +66,22 @@ class Stats(C.Structure):
                 for name, _ in self._fields_ if name != "reserved"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("swap_ratio", C.c_int32),
+        ("spec_max", C.c_int32),
+        ("filter_blocks", C.c_int32),
+        ("align_blocks", C.c_int32),
+        ("trials", C.c_int32),
+        ("reserved", C.c_int32),
+        ("ms_default", C.c_double),
+        ("ms_tuned", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 # every entry point of include/gsdr.h: (name, restype, argtypes)
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
@@ -93,6 +109,9 @@ SIGNATURES = [
     ("gsdr_cluster", C.c_int, [_P, _P, C.POINTER(Params), C.POINTER(C.c_int32), C.POINTER(Stats)]),
     ("gsdr_cluster_jobs", C.c_int, [_P, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(Params),
                                     C.POINTER(C.POINTER(C.c_int32)), C.POINTER(Stats)]),
+    ("gsdr_autotune", C.c_int, [_P, _P, C.POINTER(Params), C.c_int32, C.POINTER(Tuning)]),
+    ("gsdr_set_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
+    ("gsdr_get_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
     ("gsdr_align_pairs", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_double)]),
@@ -283,6 +302,15 @@ class Engine:
         _check(lib().gsdr_cluster_jobs(self.h, n, handles, params, ptrs, C.byref(st)), "cluster_jobs")
         return outs, st
 
+    def set_tuning(self, tuning=None):
+        """Sets the knobs of calls with tune=True (None: back to the defaults)."""
+        _check(lib().gsdr_set_tuning(self.h, C.byref(tuning) if tuning is not None else None), "set_tuning")
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().gsdr_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t
+
     def close(self):
         if self.h:
             lib().gsdr_close(self.h)
@@ -320,6 +348,18 @@ class Store:
         st = Stats()
         _check(lib().gsdr_cluster(self.engine.h, self.h, C.byref(p), _i32p(out), C.byref(st)), "cluster")
         return out, st
+
+    def autotune(self, threshold, word_length=0, spec_reps=0, max_trials=0):
+        """Times batched clusterings of this store under different launch
+        knobs and keeps the fastest in the engine (used with tune=True)."""
+        p = Params()
+        p.threshold = threshold
+        p.word_length = word_length
+        p.mode = MODE_BATCHED
+        p.spec_reps = spec_reps
+        t = Tuning()
+        _check(lib().gsdr_autotune(self.engine.h, self.h, C.byref(p), max_trials, C.byref(t)), "autotune")
+        return t
 
     def free(self):
         if self.h:

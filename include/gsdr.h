@@ -50,7 +50,7 @@ typedef struct gsdr_params {
   int32_t mode;         /* enum gsdr_mode */
   int32_t spec_reps;    /* batched: representatives evaluated per speculative step (0 = default) */
   int32_t timing;       /* 1 = time every filter/align launch with HIP events */
-  int32_t tune;         /* 1 = use the autotuned align shapes when available */
+  int32_t tune;         /* 1 = use the context's knobs from gsdr_autotune / gsdr_set_tuning */
   int32_t reserved[2];
 } gsdr_params;
 
@@ -80,6 +80,18 @@ typedef struct gsdr_stats {
   double cells_decided; /* band cells of the rows each exact decision needed (32-row steps) */
   double reserved[3];
 } gsdr_stats;
+
+/* Launch knobs that change speed, never results. */
+typedef struct gsdr_tuning {
+  int32_t swap_ratio;     /* align with the roles swapped when n_rep >= swap_ratio * m_cand (0 = never) */
+  int32_t spec_max;       /* batched: cap on speculative representatives per step (1..64) */
+  int32_t filter_blocks;  /* filter workgroups per CU */
+  int32_t align_blocks;   /* alignment workgroups per CU, per tile shape */
+  int32_t trials;         /* autotune: clusterings timed */
+  int32_t reserved;
+  double ms_default;      /* autotune: clustering time with the default knobs */
+  double ms_tuned;        /* autotune: clustering time with the chosen knobs */
+} gsdr_tuning;
 
 /* ---- records (host side): replaces readFile/saveFile (func.cpp:70-112,
  *      785-799) and the Read vector (func.h:17-20) ---- */
@@ -146,6 +158,18 @@ int gsdr_cluster(gsdr_ctx *ctx, gsdr_store *store, const gsdr_params *params,
  * counterpart; each job's result equals gsdr_cluster on that job alone). */
 int gsdr_cluster_jobs(gsdr_ctx *ctx, int32_t njobs, gsdr_store *const *stores,
                       const gsdr_params *params, int32_t *const *cluster_out, gsdr_stats *stats);
+/* Autotuner (no counterpart in the reference): clusters `store` with params
+ * (forced to the batched form) under coordinate descent over the knobs of
+ * gsdr_tuning, at most max_trials clusterings (0 = 16), and keeps the fastest
+ * set in the context; later calls with params.tune = 1 use it. Every trial
+ * must return the same clusters, else GSDR_E_DEVICE. out (may be NULL) gets
+ * the chosen knobs and timings. */
+int gsdr_autotune(gsdr_ctx *ctx, gsdr_store *store, const gsdr_params *params, int32_t max_trials,
+                  gsdr_tuning *out);
+/* Sets (tuning != NULL) or clears (NULL) the context's knobs. */
+int gsdr_set_tuning(gsdr_ctx *ctx, const gsdr_tuning *tuning);
+/* The knobs calls with params.tune = 1 use (the defaults when none are set). */
+int gsdr_get_tuning(const gsdr_ctx *ctx, gsdr_tuning *out);
 
 /* ---- pair-level entry (diagnostics and parity tests): decides
  *      LCS(text, query) > bc on the device, the test of kernel_dynamic

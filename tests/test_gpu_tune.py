@@ -1,0 +1,69 @@
+"""Autotuner: launch knobs change speed, never clusters."""
+import os
+
+import numpy as np
+import pytest
+
+import gsdr
+import oracle_ffi as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(tmp_path, recs, t):
+    fa = str(tmp_path / "in.fasta")
+    recs.write_fasta(fa)
+    ocl, _, _ = O.cluster_file(fa, t)
+    r = gsdr.Records.read_fasta(fa, length_order=True)
+    return r, ocl
+
+
+@pytest.mark.parametrize("law,n,t", [(1, 3000, 0.95), (4, 4000, 0.80)])
+def test_every_knob_setting_gives_the_oracle_clusters(engine, tmp_path, law, n, t):
+    r, ocl = _oracle(tmp_path, gsdr.Records.generate(law, 1000 + law, n), t)
+    store = engine.load(r, gsdr.auto_word_length(t))
+    try:
+        settings = [(2, 64, 8, 4), (0, 64, 8, 4), (8, 1, 1, 1), (3, 16, 16, 2), (1, 7, 2, 8)]
+        for swap, spec, fb, ab in settings:
+            tu = gsdr.Tuning(swap_ratio=swap, spec_max=spec, filter_blocks=fb, align_blocks=ab)
+            engine.set_tuning(tu)
+            got = engine.tuning()
+            assert (got.swap_ratio, got.spec_max, got.filter_blocks, got.align_blocks) == (swap, spec, fb, ab)
+            for mode in (gsdr.MODE_PLAIN, gsdr.MODE_BATCHED):
+                cl, st = store.cluster(t, store.k, mode=mode, tune=True)
+                assert np.array_equal(cl, ocl), (swap, spec, fb, ab, mode)
+                assert st.pairs == st.rej_length + st.rej_comp + st.rej_word + st.aligned
+    finally:
+        engine.set_tuning(None)
+        store.free()
+
+
+def test_autotune_then_tuned_calls_match(engine, tmp_path):
+    r, ocl = _oracle(tmp_path, gsdr.Records.generate(2, 1002, 3000), 0.90)
+    store = engine.load(r, 5)
+    try:
+        tu = store.autotune(0.90, 5, max_trials=6)
+        assert 1 <= tu.trials <= 6
+        assert tu.ms_default > 0 and 0 < tu.ms_tuned <= tu.ms_default
+        got = engine.tuning()
+        assert (got.swap_ratio, got.spec_max, got.filter_blocks, got.align_blocks) == (
+            tu.swap_ratio, tu.spec_max, tu.filter_blocks, tu.align_blocks)
+        cl, _ = store.cluster(0.90, 5, mode=gsdr.MODE_BATCHED, tune=True)
+        assert np.array_equal(cl, ocl)
+    finally:
+        engine.set_tuning(None)
+        store.free()
+    d = engine.tuning()
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks) == (4, 64, 8, 4)
+
+
+def test_tuning_out_of_range_is_rejected(engine):
+    for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0),
+                dict(align_blocks=65)]:
+        tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4)
+        for k, v in bad.items():
+            setattr(tu, k, v)
+        with pytest.raises(gsdr.GsdrError):
+            engine.set_tuning(tu)
+    d = engine.tuning()
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks) == (4, 64, 8, 4)
