@@ -550,6 +550,13 @@ __device__ inline void align_loop(const AlignArgs& a) {
       const int rows = mq - jb * 32;
       qnv = rows >= 32 ? 0u : ~((1u << rows) - 1u);
     }
+    if constexpr (S == 64) {
+      // one pair per wave: the query bits are wave-uniform, so the per-row
+      // masks are made on the scalar unit
+      qlo = __builtin_amdgcn_readfirstlane(qlo);
+      qhi = __builtin_amdgcn_readfirstlane(qhi);
+      qnv = __builtin_amdgcn_readfirstlane(qnv);
+    }
 
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
@@ -557,9 +564,10 @@ __device__ inline void align_loop(const AlignArgs& a) {
       const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
 #pragma unroll
       for (int r = 0; r < UR; ++r) {
-        const uint32_t ql = 0u - ((qlc >> r) & 1u);
-        const uint32_t qh = 0u - ((qhc >> r) & 1u);
-        const uint32_t qn = 0u - ((qnc >> r) & 1u);
+        // bit r as an all-zeros / all-ones mask (one signed bit-field extract)
+        const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
+        const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
+        const uint32_t qn = (uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
         uint32_t x[W], s[W];
         uint32_t carry = 0u, pall = ~0u;
 #pragma unroll
@@ -578,8 +586,12 @@ __device__ inline void align_loop(const AlignArgs& a) {
         // nothing, so they do not count.
         uint32_t cin;
         if (__builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0) {
+          // lane 0 of the wave keeps the old value 0; a segment's first lane
+          // of a narrower tile is cleared
           cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-          if (sl == 0) cin = 0u;
+          if constexpr (S < 64) {
+            if (sl == 0) cin = 0u;
+          }
         } else {
           const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
           const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
