@@ -104,7 +104,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     int k = 0;
     if ((st = check_params(s, params[j], &k))) return st;
     if ((st = store_set_threshold(s, params[j].threshold))) return st;
-    jobs[j] = s.job();
+    jobs[j] = s.job(params[j]);
     max_k = std::max(max_k, k);
     smax = std::max(smax, max_shape_for(s.max_m));
     total_n += s.n;

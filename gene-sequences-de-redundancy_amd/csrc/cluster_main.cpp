@@ -7,6 +7,9 @@
 //   --membership FILE      also write sorted index, original index, rep, name
 //   --report FILE          JSON report (throughput, filter rates, cells)
 //   --device N             HIP device index
+//   --drop 0|1             1 = the cuda-version's drop mode (no composition or
+//                          shared-word filter; cuda-version/func.cu:1068-1097);
+//                          results then differ from the root reference's
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +36,7 @@ struct Options {
   int mode = GSDR_MODE_BATCHED;
   int spec = 0;
   int device = 0;
+  int drop = 0;
 };
 
 Options parse(int argc, char** argv) {
@@ -48,8 +52,9 @@ Options parse(int argc, char** argv) {
       else if (key == "--membership") o.membership = val;
       else if (key == "--report") o.report = val;
       else if (key == "--device") o.device = std::atoi(val.c_str());
+      else if (key == "--drop") o.drop = (val == "1") ? 1 : (val == "0" ? 0 : -1);
       else usage();
-      if (o.mode < 0) usage();
+      if (o.mode < 0 || o.drop < 0) usage();
       i += 2;
       continue;
     }
@@ -142,6 +147,7 @@ int main(int argc, char** argv) {
   p.word_length = o.k;
   p.mode = o.mode;
   p.spec_reps = o.spec;
+  p.drop_filters = o.drop;
   std::vector<int32_t> cl((size_t)n);
   gsdr_stats stats;
   std::memset(&stats, 0, sizeof(stats));

@@ -186,7 +186,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   hipStream_t q = c->stream;
   Workspace& w = c->ws;
   const int n = s.n;
-  const JobDev jd = s.job();
+  const JobDev jd = s.job(p);
   HIP_TRY(hipMemcpyAsync(w.jobs, &jd, sizeof(JobDev), hipMemcpyHostToDevice, q), "job table");
   HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
   HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");

@@ -88,8 +88,9 @@ struct gsdr_store {
     v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
     return v;
   }
-  gsdr::JobDev job() const {
+  gsdr::JobDev job(const gsdr_params& p) const {
     gsdr::JobDev j;
+    j.flags = p.drop_filters ? gsdr::kJobDropFilters : 0;
     j.st = view();
     j.cluster = cluster;
     j.cand[0] = cand[0];

@@ -140,7 +140,13 @@ struct JobDev {
   JobState* state;
   int32_t n;
   int32_t k;                 // word length of the stored profiles
+  int32_t flags;             // kJobDropFilters
 };
+
+// Job flag: the cuda-version's drop mode (cuda-version/func.cu:1068-1097):
+// no composition or shared-word filter, every pair past the length test is
+// aligned. Opt-in; clusters then differ from the root reference's.
+constexpr int kJobDropFilters = 1;
 
 // Alignment work item: text and query sorted indices, the job and the
 // representative's slot in this step. Normally text = representative and

@@ -191,6 +191,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     }
     const int pos = p0 + chunk * kFilterChunk + grp;
     if (pos >= p1) continue;
+    const bool drop = (J.flags & kJobDropFilters) != 0;
     const int q = cand[pos];
     const int mq = J.st.m[q];
     const int bcq = J.st.bc[q];
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       }
       const uint4 cr = s_comp[g];
       const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
-      if (sum <= bcq) {
+      if (!drop && sum <= bcq) {
         if (gl == 0) ++rej_comp;
         continue;
       }
@@ -217,7 +218,8 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
     if (mask == 0) continue;
     // shared words: q's profile read once, one table lookup per survivor so far
     const int nw = 1 << (2 * J.k);
-    const int64_t e0 = J.st.kpoff[q], e1 = J.st.kpoff[q + 1];
+    // drop mode reads no profile and passes every pair
+    const int64_t e0 = J.st.kpoff[q], e1 = drop ? e0 : J.st.kpoff[q + 1];
     int acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) acc[g] = 0;
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       }
     }
     if (gl == 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
-    const int wcq = J.st.wc[q];
+    const int wcq = drop ? 0 : J.st.wc[q];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (!((mask >> g) & 1u)) continue;

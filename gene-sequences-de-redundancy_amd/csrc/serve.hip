@@ -153,7 +153,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
     std::vector<JobDev> table((size_t)nj);
     int smax = 0, max_k = 4;
     for (int i = 0; i < nj; ++i) {
-      table[i] = srv->running[i]->store->job();
+      table[i] = srv->running[i]->store->job(srv->running[i]->params);
       smax = std::max(smax, max_shape_for(srv->running[i]->store->max_m));
       max_k = std::max(max_k, srv->running[i]->store->k);
     }

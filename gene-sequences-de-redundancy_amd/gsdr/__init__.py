@@ -29,7 +29,8 @@ class Params(C.Structure):
         ("spec_reps", C.c_int32),
         ("timing", C.c_int32),
         ("tune", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("drop_filters", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -336,8 +337,10 @@ class Store:
     def footprint(self):
         return lib().gsdr_store_footprint(self.h)
 
-    def cluster(self, threshold, word_length=0, mode=MODE_PLAIN, spec_reps=0, timing=False, tune=False):
+    def cluster(self, threshold, word_length=0, mode=MODE_PLAIN, spec_reps=0, timing=False, tune=False,
+                drop_filters=False):
         p = Params()
+        p.drop_filters = 1 if drop_filters else 0
         p.threshold = threshold
         p.word_length = word_length
         p.mode = mode

@@ -51,7 +51,10 @@ typedef struct gsdr_params {
   int32_t spec_reps;    /* batched: representatives evaluated per speculative step (0 = default) */
   int32_t timing;       /* 1 = time every filter/align launch with HIP events */
   int32_t tune;         /* 1 = use the context's knobs from gsdr_autotune / gsdr_set_tuning */
-  int32_t reserved[2];
+  int32_t drop_filters; /* 1 = the cuda-version's drop mode (cuda-version/func.cu:1068-1097):
+                           no composition or shared-word filter, every pair past the length
+                           test is aligned; clusters then differ from root's. Default 0. */
+  int32_t reserved;
 } gsdr_params;
 
 typedef struct gsdr_stats {

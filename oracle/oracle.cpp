@@ -17,7 +17,8 @@
 //                           no trailing newline, header-only records, IUPAC)
 //   greedy driver           brute-force recursion (Python) on small inputs and
 //                           the idempotence known answer on the reference's
-//                           shipped result.fasta (subset fixture)
+//                           shipped result.fasta (subset fixture); its drop
+//                           mode against the same recursion without F1/F2
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -248,7 +249,10 @@ struct Stats {
 };
 
 // clustering func.cpp:677-784 with the per-round kernels func.cpp:499-676.
-void cluster_seqs(const std::vector<Seq>& seqs, int k, int threads, bool fast,
+// drop: the cuda-version's drop mode (cuda-version/func.cu:1068-1097), where
+// each round aligns every remaining sequence without the composition and
+// shared-word filters; the rest of the semantics stays root's.
+void cluster_seqs(const std::vector<Seq>& seqs, int k, int threads, bool fast, bool drop,
                   std::vector<int32_t>& cluster, Stats& st, int64_t max_rounds) {
   const int n = (int)seqs.size();
   cluster.assign(n, -1);
@@ -280,14 +284,16 @@ void cluster_seqs(const std::vector<Seq>& seqs, int k, int threads, bool fast,
     for (long long idx = 0; idx < (long long)remain.size(); ++idx) {
       const int q = remain[idx];
       const Seq& s = seqs[q];
-      // kernel_preFilter func.cpp:525-543
-      int sum = 0;
-      for (int b = 0; b < 4; ++b) sum += std::min(r.comp[b], s.comp[b]);
-      if (sum < s.bc) { rc++; continue; }
-      // kernel_filter func.cpp:544-571
-      int shared = 0;
-      for (auto& e : s.words) shared += std::min(table[e.first], e.second);
-      if (shared < s.wc) { rw++; continue; }
+      if (!drop) {
+        // kernel_preFilter func.cpp:525-543
+        int sum = 0;
+        for (int b = 0; b < 4; ++b) sum += std::min(r.comp[b], s.comp[b]);
+        if (sum < s.bc) { rc++; continue; }
+        // kernel_filter func.cpp:544-571
+        int shared = 0;
+        for (auto& e : s.words) shared += std::min(table[e.first], e.second);
+        if (shared < s.wc) { rw++; continue; }
+      }
       // kernel_dynamic func.cpp:658-675
       al++;
       bool acc = fast ? decide_fast(r, s, s.bc, nullptr) : (lcs(r, s) > s.bc);
@@ -364,7 +370,7 @@ void orc_sort_order(const int64_t* lens, int64_t n, int64_t* perm) {
 // negative value on error.
 long long orc_cluster_file(const char* in_path, const char* out_path, float t, int k, int threads,
                            int fast, int32_t* cluster_out, int64_t* perm_out, long long n_cap,
-                           long long max_rounds, double* stats_out) {
+                           long long max_rounds, double* stats_out, int drop) {
   std::vector<Read> reads;
   if (!read_fasta(in_path, reads)) return -1;
   if (reads.empty()) return -2;
@@ -381,7 +387,7 @@ long long orc_cluster_file(const char* in_path, const char* out_path, float t, i
   for (long long i = 0; i < (long long)n; ++i) prepare(sorted[i].data, k, t, seqs[i]);
   std::vector<int32_t> cluster;
   Stats st;
-  cluster_seqs(seqs, k, threads, fast != 0, cluster, st, max_rounds);
+  cluster_seqs(seqs, k, threads, fast != 0, drop != 0, cluster, st, max_rounds);
   if (out_path && out_path[0]) {
     std::ofstream f(out_path, std::ios::binary);
     for (int64_t s = 0; s < n; ++s)
@@ -431,7 +437,7 @@ int main(int argc, char** argv) {
   int fast = argc > 6 ? atoi(argv[6]) : 1;
   std::vector<int32_t> cl(1 << 26);
   double st[8] = {0};
-  long long n = orc_cluster_file(argv[1], argv[2], t, k, threads, fast, cl.data(), nullptr, (long long)cl.size(), -1, st);
+  long long n = orc_cluster_file(argv[1], argv[2], t, k, threads, fast, cl.data(), nullptr, (long long)cl.size(), -1, st, 0);
   if (n < 0) { fprintf(stderr, "error %lld\n", n); return 1; }
   if (argc > 7) { FILE* f = fopen(argv[7], "wb"); fwrite(cl.data(), 4, (size_t)n, f); fclose(f); }
   printf("{\"n\": %lld, \"reps\": %.0f, \"pairs\": %.0f, \"rej_comp\": %.0f, \"rej_word\": %.0f, \"aligned\": %.0f, \"accepted\": %.0f, \"seconds\": %.3f}\n",

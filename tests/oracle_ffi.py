@@ -35,7 +35,7 @@ def orc():
         L.orc_sort_order.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_cluster_file.restype = C.c_longlong
         L.orc_cluster_file.argtypes = [C.c_char_p, C.c_char_p, C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p,
-                                       C.c_void_p, C.c_longlong, C.c_longlong, C.c_void_p]
+                                       C.c_void_p, C.c_longlong, C.c_longlong, C.c_void_p, C.c_int]
         L.orc_read_fasta.restype = C.c_longlong
         L.orc_read_fasta.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong,
                                      C.c_longlong, C.POINTER(C.c_longlong)]
@@ -90,8 +90,9 @@ def sort_order(lens):
     return perm
 
 
-def cluster_file(path, t, k=0, threads=8, fast=True, out_path="", max_rounds=-1):
-    """Oracle run of the whole path on a FASTA file.
+def cluster_file(path, t, k=0, threads=8, fast=True, out_path="", max_rounds=-1, drop=False):
+    """Oracle run of the whole path on a FASTA file (drop: no composition or
+    shared-word filter, the cuda-version's drop mode).
 
     Returns (cluster[] over sorted indices, perm, stats dict)."""
     n_cap = 1 << 22
@@ -99,7 +100,8 @@ def cluster_file(path, t, k=0, threads=8, fast=True, out_path="", max_rounds=-1)
     perm = np.zeros(n_cap, dtype=np.int64)
     st = np.zeros(8, dtype=np.float64)
     n = orc().orc_cluster_file(os.fsencode(path), os.fsencode(out_path), t, k, threads, 1 if fast else 0,
-                               cl.ctypes.data, perm.ctypes.data, n_cap, max_rounds, st.ctypes.data)
+                               cl.ctypes.data, perm.ctypes.data, n_cap, max_rounds, st.ctypes.data,
+                               1 if drop else 0)
     if n < 0:
         raise RuntimeError(f"oracle failed on {path}: {n}")
     keys = ["reps", "pairs", "rej_comp", "rej_word", "aligned", "accepted", "seconds"]

@@ -37,6 +37,25 @@ def test_plain_matches_oracle(engine, tmp_path, config, n, t):
     assert st.aligned == st.accepted + st.rej_align
 
 
+@pytest.mark.parametrize("config,n,t", [(1, 3000, 0.95), (2, 2000, 0.90), (4, 4000, 0.80)])
+def test_drop_mode_matches_oracle(engine, tmp_path, config, n, t):
+    """Opt-in drop mode (no composition or shared-word filter) in both forms
+    against the oracle's drop mode."""
+    recs = gsdr.Records.generate(config, 1000 + config, n)
+    fa = str(tmp_path / "in.fasta")
+    recs.write_fasta(fa)
+    ocl, operm, ost = O.cluster_file(fa, t, drop=True)
+    r = gsdr.Records.read_fasta(fa, length_order=True)
+    assert np.array_equal(r.sort(), operm)
+    for mode in (gsdr.MODE_PLAIN, gsdr.MODE_BATCHED):
+        cl, st = gsdr.cluster_records(engine, r, t, mode=mode, drop_filters=True)
+        bad = np.nonzero(cl != ocl)[0]
+        assert len(bad) == 0, f"{len(bad)} assignments differ, first at {bad[:5]}"
+        assert st.rej_comp == 0 and st.rej_word == 0
+        assert st.pairs == st.rej_length + st.aligned
+        assert st.reps == ost["reps"]
+
+
 def test_iupac_crlf_edge_set(engine, tmp_path):
     rng = np.random.default_rng(5)
     lines = []

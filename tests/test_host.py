@@ -134,7 +134,8 @@ def run_cli(*args, cwd=None):
 
 def test_cli_usage_errors_exit_zero(tmp_path):
     # odd argument count, unknown option, t out of range, w out of range: usage, exit 0
-    for args in (("t",), ("x", "1"), ("t", "0.7"), ("t", "1.0"), ("w", "3"), ("w", "9"), ("t", "abc")):
+    for args in (("t",), ("x", "1"), ("t", "0.7"), ("t", "1.0"), ("w", "3"), ("w", "9"), ("t", "abc"),
+                 ("--drop", "2"), ("d", "1"), ("p", "1")):
         p = run_cli(*args, cwd=tmp_path)
         assert p.returncode == 0, args
         assert "use like this" in p.stdout, args

@@ -244,8 +244,9 @@ def test_fasta_edge_cases(tmp_path, name):
 
 
 # ---------------------------------------------------------- greedy driver
-def brute_cluster(seqs, t):
-    """Greedy recursion of func.cpp:466-784 restated from the definitions."""
+def brute_cluster(seqs, t, drop=False):
+    """Greedy recursion of func.cpp:466-784 restated from the definitions
+    (drop: without the composition and shared-word filters)."""
     t32 = np.float32(t)
     k = 4 if t32 < 0.88 else 5 if t32 < 0.94 else 6 if t32 < 0.97 else 7
     order = O.sort_order([len(s) for s in seqs])
@@ -262,6 +263,8 @@ def brute_cluster(seqs, t):
 
     def P(r, q):
         bc, wc = cut[q]
+        if drop:
+            return dp_lcs(S[r], S[q]) > bc
         if int(np.minimum(comp[r], comp[q]).sum()) < bc:
             return False
         if sum(min(v, kc[r][w]) for w, v in kc[q].items()) < wc:
@@ -311,3 +314,18 @@ def test_greedy_brute_force(tmp_path, t):
         cl, perm, _ = O.cluster_file(str(fa), t, fast=fast)
         assert np.array_equal(perm, order)
         assert np.array_equal(cl, expect)
+
+
+@pytest.mark.parametrize("t", [0.95, 0.8])
+def test_greedy_brute_force_drop_mode(tmp_path, t):
+    """Drop mode: the shared-word filter is not sound for LCS, so leaving it
+    out can absorb more; the recursion with A alone pins the oracle's mode."""
+    seqs = family_set(int(t * 100) + 7, 6, 5, 40, 160, 0.12)
+    fa = tmp_path / "f.fasta"
+    fa.write_bytes(b"".join(b">s%d\n%s\n" % (i, s) for i, s in enumerate(seqs)))
+    expect, order = brute_cluster(seqs, t, drop=True)
+    for fast in (True, False):
+        cl, perm, st = O.cluster_file(str(fa), t, fast=fast, drop=True)
+        assert np.array_equal(perm, order)
+        assert np.array_equal(cl, expect)
+        assert st["rej_comp"] == 0 and st["rej_word"] == 0
