@@ -100,8 +100,14 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
 //     min(c_r(w), c_q(w)) < wc_q rejects (one LDS lookup per representative
 //     of the group that passed the checks above).
 // Survivors are appended to the alignment list of their tile shape.
+// Waves per SIMD the register allocation must allow: the tables take 32 KiB
+// of LDS per block, so four blocks (four waves per SIMD) fit a CU; the
+// widest group is held to that too.
+__host__ __device__ constexpr int filter_waves_per_eu(int g) { return g >= 8 ? 4 : 1; }
+
 template <int G>
-__global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
+__global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(FilterArgs a) {
+  static_assert(G >= 1 && G <= 16, "one lane of a 16-lane group per representative");
   HIP_DYNAMIC_SHARED(uint32_t, lds)
   uint16_t* table = reinterpret_cast<uint16_t*>(lds);
   __shared__ int s_pre[kMaxJobsPerLaunch + 1];
@@ -189,20 +195,22 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       cj = job;
       cg = gi;
     }
+    // every lane of the block runs the whole unit (no early exit): the
+    // survivor append below is a wave-wide operation
     const int pos = p0 + chunk * kFilterChunk + grp;
-    if (pos >= p1) continue;
+    const bool inpos = pos < p1;
     const bool drop = (J.flags & kJobDropFilters) != 0;
-    const int q = cand[pos];
+    const int q = inpos ? cand[pos] : 0;
     const int mq = J.st.m[q];
     const int bcq = J.st.bc[q];
     const int c = bcq + 1;
     const uint4 cq = J.st.comp[q];
-    if (gl == 0) bytes += 28;  // id, m, bc, comp
+    if (gl == 0 && inpos) bytes += 28;  // id, m, bc, comp
     // length and composition per representative of the group
     uint32_t mask = 0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      if (g >= gcount || pos <= s_slot[g]) continue;
+      if (!inpos || g >= gcount || pos <= s_slot[g]) continue;
       if (c > min(s_n[g], mq)) {
         if (gl == 0) ++rej_len;
         continue;
@@ -215,27 +223,29 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
       }
       mask |= 1u << g;
     }
-    if (mask == 0) continue;
-    // shared words: q's profile read once, one table lookup per survivor so far
+    // shared words: q's profile read once, one table lookup per survivor so
+    // far (none when no representative is left; drop mode reads no profile
+    // and passes every pair)
     const int nw = 1 << (2 * J.k);
-    // drop mode reads no profile and passes every pair
-    const int64_t e0 = J.st.kpoff[q], e1 = drop ? e0 : J.st.kpoff[q + 1];
+    const int64_t e0 = J.st.kpoff[q], e1 = (drop || mask == 0) ? e0 : J.st.kpoff[q + 1];
     int acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) acc[g] = 0;
-    // the profile loads are latency bound: 16, then 4, in flight per lane
+    // the profile loads are latency bound: NB (16, or 8 beside 16 tables,
+    // which need the registers), then 4, in flight per lane
+    constexpr int NB = G >= 16 ? 8 : 16;
     int64_t e = e0 + gl;
-    for (; e + 15 * 16 < e1; e += 16 * 16) {
-      uint32_t xs[16];
+    for (; e + (NB - 1) * 16 < e1; e += NB * 16) {
+      uint32_t xs[NB];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) xs[i] = J.st.kp[e + 16 * i];
+      for (int i = 0; i < NB; ++i) xs[i] = J.st.kp[e + 16 * i];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (!((mask >> g) & 1u)) continue;
         const uint16_t* tg = table + g * nw;
         int sg = 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
+        for (int i = 0; i < NB; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
         acc[g] += sg;
       }
     }
@@ -257,40 +267,60 @@ __global__ __launch_bounds__(256) void filter_kernel(FilterArgs a) {
         acc[g] += min((uint32_t)table[g * nw + (x & 0xFFFFu)], x >> 16);
       }
     }
-    if (gl == 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
+    if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
     const int wcq = drop ? 0 : J.st.wc[q];
+    // per representative g of the group: the 16-lane total, kept by lane g
+    int mine = 0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (!((mask >> g) & 1u)) continue;
       int sg = acc[g];
 #pragma unroll
       for (int off = 8; off > 0; off >>= 1) sg += __shfl_xor(sg, off, 16);
-      if (gl != 0) continue;
-      if (sg < wcq) {
-        ++rej_word;
-        continue;
-      }
-      const int r = s_rep[g], slot = s_slot[g], n = s_n[g];
+      if (gl == g) mine = sg;
+    }
+    // lane g decides representative g; survivors of the wave are appended
+    // with one list reservation per tile shape present
+    const bool live = gl < G && ((mask >> gl) & 1u);
+    const bool pass = live && mine >= wcq;
+    if (live && !pass) ++rej_word;
+    int sh = 0;
+    AlignItem it{0, 0, 0, 0};
+    if (pass) {
+      const int r = s_rep[gl], slot = s_slot[gl], n = s_n[gl];
       ++passed;
       bytes += 16;
       cband += (unsigned long long)band_cells(n, mq, c);
       cfull += (unsigned long long)n * (unsigned long long)mq;
       const bool sw = swap_roles(n, mq, a.swap_ratio);
-      const int sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
-      const int at = atomicAdd(&a.bin_count[sh], 1);
-      AlignItem it;
+      sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
       it.text = sw ? q : r;
       it.query = sw ? r : q;
       it.job = job;
       it.slot = slot | (sw ? kSwapFlag : 0);
-      if (at < a.item_cap) {
-        a.items[(int64_t)sh * a.item_cap + at] = it;
-      } else {
-        // the list is full: count it so the host fails the call (the host
-        // bounds k_spec * nc so that this does not happen)
-        atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
-        atomicSub(&a.bin_count[sh], 1);
+    }
+    const int wl = tid & 63;
+    bool todo = pass;
+    for (uint64_t pend = __builtin_amdgcn_ballot_w64(todo); pend != 0;) {
+      const int l = (int)__builtin_ctzll(pend);
+      const int s0 = __shfl(sh, l);
+      const uint64_t same = __builtin_amdgcn_ballot_w64(todo && sh == s0);
+      int base = 0;
+      if (wl == l) base = atomicAdd(&a.bin_count[s0], (int)__popcll(same));
+      base = __shfl(base, l);
+      if (todo && sh == s0) {
+        const int at = base + (int)__popcll(same & ((1ull << wl) - 1ull));
+        if (at < a.item_cap) {
+          a.items[(int64_t)sh * a.item_cap + at] = it;
+        } else {
+          // the list is full: count it so the host fails the call (the
+          // host bounds k_spec * nc so that this does not happen)
+          atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
+          atomicSub(&a.bin_count[sh], 1);
+        }
+        todo = false;
       }
+      pend &= ~same;
     }
   }
   unsigned long long v[7] = {rej_len, rej_comp, rej_word, passed, bytes, cband, cfull};
