@@ -485,10 +485,14 @@ __device__ inline void align_loop(const AlignArgs& a) {
     // representative of this step) are settled here and the fetch repeats
     while (true) {
       const bool need = (item < 0) && !exhausted;
-      if (__builtin_amdgcn_ballot_w64(need) == 0) break;
-      int got = 0;
-      if (need && sl == 0) got = atomicAdd(a.work, 1);
-      got = __shfl(got, seg0);
+      // one work-counter reservation for every segment of the wave that
+      // needs a pair; segment i of them takes the i-th index
+      const uint64_t needm = __builtin_amdgcn_ballot_w64(need && sl == 0);
+      if (needm == 0) break;
+      int base = 0;
+      if (lane == 0) base = atomicAdd(a.work, (int)__popcll(needm));
+      base = __shfl(base, 0);
+      const int got = base + (int)__popcll(needm & ((1ull << seg0) - 1ull));
       if (!need) continue;
       if (got >= total) {
         exhausted = true;
