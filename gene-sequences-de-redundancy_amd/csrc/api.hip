@@ -271,13 +271,15 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
     return GSDR_OK;
   }
   if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || t->filter_blocks < 1 ||
-      t->filter_blocks > 64 || t->align_blocks < 1 || t->align_blocks > 64)
+      t->filter_blocks > 64 || t->align_blocks < 0 || t->align_blocks > 64 ||
+      (t->fork_shapes != 0 && t->fork_shapes != 1))
     return fail(GSDR_E_INVALID, "tuning out of range");
   Tuning T;
   T.swap_ratio = t->swap_ratio;
   T.spec_max = t->spec_max;
   T.filter_blocks = t->filter_blocks;
   T.align_blocks = t->align_blocks;
+  T.fork_shapes = t->fork_shapes;
   c->tuned = T;
   c->has_tuning = true;
   return GSDR_OK;
@@ -291,6 +293,7 @@ int gsdr_get_tuning(const gsdr_ctx* c, gsdr_tuning* out) {
   out->spec_max = T.spec_max;
   out->filter_blocks = T.filter_blocks;
   out->align_blocks = T.align_blocks;
+  out->fork_shapes = T.fork_shapes;
   return GSDR_OK;
 }
 

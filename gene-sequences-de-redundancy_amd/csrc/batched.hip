@@ -29,13 +29,13 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   if (ev && (e = hipEventRecord(ev[0], q)) != hipSuccess) return e;
   if ((e = launch_filter(L.fa_reps, L.max_k, L.reps_filter_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
-  if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
   if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
   if ((e = launch_filter(L.fa_main, L.max_k, L.filter_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
-  if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
   return launch_resolve(L.ra, q);
 }
@@ -45,10 +45,11 @@ StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t
   StepLaunch L;
   L.max_k = max_k;
   L.smax = smax;
+  L.fork = T.fork_shapes;
   L.filter_grid = c->cu_count * T.filter_blocks;
   L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
   for (int sh = 0; sh < kNumShapes; ++sh) {
-    L.align_grid[sh] = c->cu_count * T.align_blocks;
+    L.align_grid[sh] = c->cu_count * align_blocks_for(T, sh);
     // phase A holds at most k_spec^2/2 pairs per job
     const int segs = 64 / kShapes[sh].s;
     L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));

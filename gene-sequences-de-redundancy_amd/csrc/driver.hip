@@ -149,8 +149,14 @@ int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   return cluster_plain(c, s, p, out, stats);
 }
 
-hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main) {
+hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main,
+                               int fork) {
   hipError_t e;
+  if (!fork) {
+    for (int sh = 0; sh <= smax; ++sh)
+      if ((e = launch_align(sh, args[sh], grids[sh], main)) != hipSuccess) return e;
+    return hipSuccess;
+  }
   if ((e = hipEventRecord(c->fork, main)) != hipSuccess) return e;
   for (int sh = 0; sh <= smax; ++sh) {
     if ((e = hipStreamWaitEvent(c->side[sh], c->fork, 0)) != hipSuccess) return e;
@@ -200,7 +206,6 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   int nc = n, first = 0;
   bool done = n == 0;
   const Tuning T = tuning_for(c, p.tune != 0);
-  const int cap_blocks = c->cu_count * T.align_blocks;
   while (!done) {
     ++rounds;
     const int units = (nc + kFilterChunk - 1) / kFilterChunk;
@@ -232,11 +237,11 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       aa[sh].skip_absorbed = 0;
       const int segs = 64 / kShapes[sh].s;
       const int waves = (nc + segs - 1) / segs;
-      grids[sh] = std::max(1, std::min((waves + 3) / 4, cap_blocks));
+      grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh)));
       ++launches;
       ++align_launches;
     }
-    HIP_TRY(launch_align_shapes(c, aa, smax, grids, q), "align launch");
+    HIP_TRY(launch_align_shapes(c, aa, smax, grids, q, T.fork_shapes), "align launch");
     if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
     ResolveArgs ra;
     ra.jobs = w.jobs;
@@ -393,7 +398,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     aa.skip_absorbed = 0;
     const int segs = 64 / kShapes[sh].s;
     const int waves = (counts[sh] + segs - 1) / segs;
-    const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * 4));
+    const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_per_cu(sh)));
     if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return bail(e, "align launch");
   }
   std::vector<int32_t> got(all.size());

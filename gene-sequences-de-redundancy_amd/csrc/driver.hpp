@@ -35,7 +35,8 @@ struct Tuning {
   int swap_ratio = gsdr::kSwapRatio;
   int spec_max = gsdr::kMaxSpec;
   int filter_blocks = 8;
-  int align_blocks = 4;
+  int align_blocks = 4;  // 0: each shape kernel's resident blocks per CU
+  int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
 };
 
 struct gsdr_ctx {
@@ -122,7 +123,10 @@ int max_shape_for(int n);
 struct AlignArgs;
 // Launches the alignment kernels of shapes 0..smax concurrently on the side
 // streams, forked from and joined back into `main`.
-hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main);
+// fork = 1: the shape kernels run concurrently on the side streams (fork /
+// join events); 0: one after another on `main` (no cross-queue waits).
+hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main,
+                               int fork = 1);
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
 int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
@@ -146,6 +150,7 @@ struct StepLaunch {
   AlignArgs aa[kNumShapes];
   int smax = 0;
   int max_k = 4;
+  int fork = 1;  // shape kernels on side streams
   int filter_grid = 1;
   int align_grid[kNumShapes] = {};
   int reps_grid[kNumShapes] = {};
@@ -153,6 +158,14 @@ struct StepLaunch {
 };
 StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left,
                             const Tuning& T = Tuning());
+// Alignment workgroups per CU for a tile shape under knobs T; 0 = the
+// shape kernel's resident blocks per CU (registers and LDS decide it).
+// Measured: resident grids help config 2 (many short pairs, 910 -> 776 ms)
+// and cost config 4 2 % (its long-pair kernel, which ends each step, then
+// shares the CUs with more blocks of the other shapes), so the default is 4.
+inline int align_blocks_for(const Tuning& T, int shape) {
+  return T.align_blocks > 0 ? T.align_blocks : gsdr::align_blocks_per_cu(shape);
+}
 // The knobs a call runs with: the context's tuned ones when asked for and present.
 inline Tuning tuning_for(const gsdr_ctx* c, bool want) { return want && c->has_tuning ? c->tuned : Tuning(); }
 int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int max_trials, gsdr_tuning* out);

@@ -234,6 +234,38 @@ static hipError_t launch_align_sw(const AlignArgs& a, int grid, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int S, int W>
+static int occupancy_sw() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)align_kernel<S, W>, 256, 0) != hipSuccess)
+    return 4;
+  return nb > 0 ? nb : 1;
+}
+
+// Resident 256-thread blocks per CU of each tile-shape kernel (registers and
+// LDS decide it), asked once: a grid of that many blocks per CU fills the
+// chip without queueing blocks that would only find the work list empty.
+int align_blocks_per_cu(int shape) {
+  static int occ[kNumShapes] = {};
+  static bool ready = false;
+  if (!ready) {
+    occ[0] = occupancy_sw<8, 1>();
+    occ[1] = occupancy_sw<16, 1>();
+    occ[2] = occupancy_sw<32, 1>();
+    occ[3] = occupancy_sw<64, 1>();
+    occ[4] = occupancy_sw<64, 2>();
+    occ[5] = occupancy_sw<64, 3>();
+    occ[6] = occupancy_sw<64, 4>();
+    occ[7] = occupancy_sw<64, 6>();
+    occ[8] = occupancy_sw<64, 8>();
+    occ[9] = occupancy_sw<64, 10>();
+    occ[10] = occupancy_sw<64, 12>();
+    occ[11] = occupancy_sw<64, 16>();
+    ready = true;
+  }
+  return shape >= 0 && shape < kNumShapes ? occ[shape] : 1;
+}
+
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s) {
   switch (shape) {
     case 0: return launch_align_sw<8, 1>(a, grid, s);

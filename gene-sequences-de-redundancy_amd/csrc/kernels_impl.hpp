@@ -1018,6 +1018,7 @@ __global__ __launch_bounds__(256, align_waves_per_eu(S, W)) void align_kernel(Al
 // host launchers (kernels.hip)
 hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s);
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
+int align_blocks_per_cu(int shape);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s);

@@ -1,6 +1,6 @@
 // Autotuner for the launch knobs of the batched form: role-swap ratio, cap
 // on speculative representatives per step, filter and alignment workgroups
-// per CU. None of them changes a result (every setting computes the same
+// per CU, and whether the tile-shape kernels run concurrently. None of them changes a result (every setting computes the same
 // greedy clustering), so the tuner times whole clusterings of the caller's
 // store and keeps the fastest setting; it also checks that every trial
 // returned the same clusters. No counterpart in the reference, whose launch
@@ -63,8 +63,9 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
   const Knob knobs[] = {
       {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
       {&Tuning::spec_max, {16, 32}},
-      {&Tuning::align_blocks, {2, 8}},
+      {&Tuning::align_blocks, {2, 8, 0}},
       {&Tuning::filter_blocks, {4, 16}},
+      {&Tuning::fork_shapes, {0}},
   };
   for (const Knob& k : knobs) {
     for (int v : k.values) {
@@ -84,7 +85,8 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
   // confirm a changed setting once more against the defaults
   if (trials < max_trials &&
       (best.swap_ratio != Tuning().swap_ratio || best.spec_max != Tuning().spec_max ||
-       best.align_blocks != Tuning().align_blocks || best.filter_blocks != Tuning().filter_blocks)) {
+       best.align_blocks != Tuning().align_blocks || best.filter_blocks != Tuning().filter_blocks ||
+       best.fork_shapes != Tuning().fork_shapes)) {
     double again = 0;
     if ((st = run(best, &again, false))) return finish(st);
     ++trials;
@@ -103,6 +105,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
     out->spec_max = best.spec_max;
     out->filter_blocks = best.filter_blocks;
     out->align_blocks = best.align_blocks;
+    out->fork_shapes = best.fork_shapes;
     out->trials = trials;
     out->ms_default = ms_default;
     out->ms_tuned = ms_best;

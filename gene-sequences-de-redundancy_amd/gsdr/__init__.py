@@ -74,13 +74,13 @@ class Tuning(C.Structure):
         ("filter_blocks", C.c_int32),
         ("align_blocks", C.c_int32),
         ("trials", C.c_int32),
-        ("reserved", C.c_int32),
+        ("fork_shapes", C.c_int32),
         ("ms_default", C.c_double),
         ("ms_tuned", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 # every entry point of include/gsdr.h: (name, restype, argtypes)

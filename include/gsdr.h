@@ -89,9 +89,10 @@ typedef struct gsdr_tuning {
   int32_t swap_ratio;     /* align with the roles swapped when n_rep >= swap_ratio * m_cand (0 = never) */
   int32_t spec_max;       /* batched: cap on speculative representatives per step (1..64) */
   int32_t filter_blocks;  /* filter workgroups per CU */
-  int32_t align_blocks;   /* alignment workgroups per CU, per tile shape */
+  int32_t align_blocks;   /* alignment workgroups per CU, per tile shape (default 4; 0 = each
+                             shape kernel's resident blocks per CU) */
   int32_t trials;         /* autotune: clusterings timed */
-  int32_t reserved;
+  int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
 } gsdr_tuning;
