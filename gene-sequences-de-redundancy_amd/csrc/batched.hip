@@ -27,23 +27,23 @@ namespace gsdr {
 hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev) {
   hipError_t e;
   if (ev && (e = hipEventRecord(ev[0], q)) != hipSuccess) return e;
-  if ((e = launch_filter(L.fa_reps, L.max_k, L.reps_filter_grid, q)) != hipSuccess) return e;
+  if ((e = launch_filter(L.fa_reps, L.kmask, L.reps_filter_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
   if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
   if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
-  if ((e = launch_filter(L.fa_main, L.max_k, L.filter_grid, q)) != hipSuccess) return e;
+  if ((e = launch_filter(L.fa_main, L.kmask, L.filter_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
   if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
   return launch_resolve(L.ra, q);
 }
 
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left, const Tuning& T) {
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, int32_t* jobs_left, const Tuning& T) {
   Workspace& w = c->ws;
   StepLaunch L;
-  L.max_k = max_k;
+  L.kmask = kmask;
   L.smax = smax;
   L.fork = T.fork_shapes;
   L.filter_grid = c->cu_count * T.filter_blocks;
@@ -96,7 +96,8 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   if (njobs > kMaxJobsPerLaunch) return fail(GSDR_E_LIMIT, "more than 1024 jobs in one call");
   HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
   int st;
-  int max_k = 4, smax = 0, timing = 0, spec = 0, tune = 0;
+  int smax = 0, timing = 0, spec = 0, tune = 0;
+  unsigned kmask = 0;
   int64_t total_n = 0, total_bases = 0, max_n = 0;
   std::vector<JobDev> jobs((size_t)njobs);
   for (int j = 0; j < njobs; ++j) {
@@ -106,7 +107,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     if ((st = check_params(s, params[j], &k))) return st;
     if ((st = store_set_threshold(s, params[j].threshold))) return st;
     jobs[j] = s.job(params[j]);
-    max_k = std::max(max_k, k);
+    kmask |= 1u << k;
     smax = std::max(smax, max_shape_for(s.max_m));
     total_n += s.n;
     total_bases += s.bases;
@@ -122,7 +123,10 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   const int k_max = std::min(spec > 0 ? std::min(spec, kMaxSpec) : kMaxSpec, T.spec_max);
   // alignment list capacity per shape: enough for several speculative
   // representatives over every job's candidates
-  const int64_t item_cap = std::min<int64_t>(std::max<int64_t>(8 * total_n, max_n), 32ll << 20);
+  // (the resolve step bounds each job's k_spec so that k_spec * nc of all
+  // jobs fits, so the capacity also caps the speculation width)
+  const int64_t item_cap =
+      std::min<int64_t>(std::max<int64_t>((int64_t)T.list_factor * total_n, max_n), 64ll << 20);
   if ((st = workspace_reserve(c, item_cap, njobs))) return st;
   // the first step queues every job's whole candidate list once
   if (total_n > c->ws.item_cap) return fail(GSDR_E_LIMIT, "too many sequences in one call");
@@ -137,7 +141,10 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs.data(), njobs, k_max, w.jobs_left, q), "init");
 
-  const StepLaunch L = make_step_launch(c, njobs, smax, max_k, w.jobs_left, T);
+  StepLaunch L = make_step_launch(c, njobs, smax, kmask, w.jobs_left, T);
+  // the speculation bound follows this call's capacity, not a larger
+  // workspace left by an earlier call
+  L.ra.pair_budget = std::min<int64_t>(item_cap, w.item_cap);
 
   double ms_filter = 0, ms_align = 0;
   int64_t launches = 2, steps_launched = 0;

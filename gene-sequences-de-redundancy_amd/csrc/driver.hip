@@ -220,7 +220,8 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     fa.item_cap = w.item_cap;
     fa.bin_count = w.counters;
     fa.stats = w.stats;
-    HIP_TRY(launch_filter(fa, k, std::max(1, std::min(units, c->cu_count * T.filter_blocks)), q), "filter launch");
+    HIP_TRY(launch_filter(fa, 1u << k, std::max(1, std::min(units, c->cu_count * T.filter_blocks)), q),
+            "filter launch");
     ++launches;
     ++filter_launches;
     if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");

@@ -37,6 +37,7 @@ struct Tuning {
   int filter_blocks = 8;
   int align_blocks = 4;  // 0: each shape kernel's resident blocks per CU
   int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
+  int list_factor = 8;   // alignment list capacity per shape: list_factor items per sequence
 };
 
 struct gsdr_ctx {
@@ -149,14 +150,14 @@ struct StepLaunch {
   AlignArgs aa_reps[kNumShapes];
   AlignArgs aa[kNumShapes];
   int smax = 0;
-  int max_k = 4;
+  unsigned kmask = 0;  // word lengths among the jobs (bit k)
   int fork = 1;  // shape kernels on side streams
   int filter_grid = 1;
   int align_grid[kNumShapes] = {};
   int reps_grid[kNumShapes] = {};
   int reps_filter_grid = 1;
 };
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, int max_k, int32_t* jobs_left,
+StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, int32_t* jobs_left,
                             const Tuning& T = Tuning());
 // Alignment workgroups per CU for a tile shape under knobs T; 0 = the
 // shape kernel's resident blocks per CU (registers and LDS decide it).

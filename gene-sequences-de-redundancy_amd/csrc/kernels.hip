@@ -216,9 +216,7 @@ static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipS
   return hipGetLastError();
 }
 
-// The group size the kernel uses is filter_group(max_k); phase_units and
-// every job's stored profiles must be for k <= max_k.
-hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+static hipError_t launch_filter_k(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
   switch (filter_group(max_k)) {
     case 16: return launch_filter_g<16>(a, max_k, grid, s);
     case 8: return launch_filter_g<8>(a, max_k, grid, s);
@@ -226,6 +224,29 @@ hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s
     case 2: return launch_filter_g<2>(a, max_k, grid, s);
     default: return launch_filter_g<1>(a, max_k, grid, s);
   }
+}
+
+// Jobs of different word lengths sharing one launch would all get the
+// table group of the longest k (one table per block for k = 7); instead each
+// run of word lengths with the same group size gets its own launch, whose
+// tables are sized for the longest k of that run.
+hipError_t launch_filter(const FilterArgs& a, unsigned kmask, int grid, hipStream_t s) {
+  int k = 0;
+  while (k < 32 && !((kmask >> k) & 1u)) ++k;
+  while (k < 32) {
+    const int g = filter_group(k);
+    int hi = k;
+    for (int k2 = k + 1; k2 < 32 && filter_group(k2) == g; ++k2)
+      if ((kmask >> k2) & 1u) hi = k2;
+    FilterArgs b = a;
+    b.k_lo = k;
+    b.k_hi = hi;
+    hipError_t e = launch_filter_k(b, hi, grid, s);
+    if (e != hipSuccess) return e;
+    k = hi + 1;
+    while (k < 32 && !((kmask >> k) & 1u)) ++k;
+  }
+  return hipSuccess;
 }
 
 template <int S, int W>

@@ -11,6 +11,7 @@ struct FilterArgs {
   int32_t njobs;
   int32_t min_shape;
   int32_t swap_ratio;       // role swap when n_rep >= swap_ratio * m_cand (0 = never)
+  int32_t k_lo, k_hi;       // this launch filters the jobs with word length in [k_lo, k_hi]
   int32_t phase;            // FilterPhase
   AlignItem* items;         // per-shape lists: items + shape * item_cap
   int64_t item_cap;
@@ -123,7 +124,10 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     for (int base = 0; base < nj; base += 256) {
       const int j = base + tid;
       int u = 0;
-      if (j < nj) u = phase_units(*a.jobs[j].state, a.phase, G);
+      if (j < nj) {
+        const JobDev& J = a.jobs[j];
+        u = (J.k >= a.k_lo && J.k <= a.k_hi) ? phase_units(*J.state, a.phase, G) : 0;
+      }
       int tot;
       const int ex = block_exclusive_scan<256>(u, &tot, s_part);
       if (j < nj) s_pre[j] = carry + ex;
@@ -1016,7 +1020,9 @@ __global__ __launch_bounds__(256, align_waves_per_eu(S, W)) void align_kernel(Al
 #endif  // GSDR_DEVICE_TU
 
 // host launchers (kernels.hip)
-hipError_t launch_filter(const FilterArgs& a, int max_k, int grid, hipStream_t s);
+// kmask: bit k set for every word length k among the jobs; one launch per
+// class of word lengths that share a table group size.
+hipError_t launch_filter(const FilterArgs& a, unsigned kmask, int grid, hipStream_t s);
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
 int align_blocks_per_cu(int shape);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);

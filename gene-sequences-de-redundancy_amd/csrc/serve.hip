@@ -151,11 +151,12 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
   if (nj > 0) {
     Workspace& w = c->ws;
     std::vector<JobDev> table((size_t)nj);
-    int smax = 0, max_k = 4;
+    int smax = 0;
+    unsigned kmask = 0;
     for (int i = 0; i < nj; ++i) {
       table[i] = srv->running[i]->store->job(srv->running[i]->params);
       smax = std::max(smax, max_shape_for(srv->running[i]->store->max_m));
-      max_k = std::max(max_k, srv->running[i]->store->k);
+      kmask |= 1u << srv->running[i]->store->k;
     }
     if ((e = hipMemcpyAsync(w.jobs, table.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice, q)) != hipSuccess)
       return hip_fail(e, "job table");
@@ -167,7 +168,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
         return hip_fail(e, "init");
     }
     if ((e = launch_clamp_spec(w.jobs, nj, w.item_cap, q)) != hipSuccess) return hip_fail(e, "clamp");
-    const StepLaunch L = make_step_launch(c, nj, smax, max_k, nullptr);
+    const StepLaunch L = make_step_launch(c, nj, smax, kmask, nullptr);
     for (int s = 0; s < srv->steps_per_call; ++s)
       if ((e = launch_step(c, L, q, nullptr)) != hipSuccess) return hip_fail(e, "serve step");
     std::vector<JobState> hs((size_t)nj);

@@ -1,6 +1,7 @@
 // Autotuner for the launch knobs of the batched form: role-swap ratio, cap
 // on speculative representatives per step, filter and alignment workgroups
-// per CU, and whether the tile-shape kernels run concurrently. None of them changes a result (every setting computes the same
+// per CU, whether the tile-shape kernels run concurrently, and the alignment
+// list capacity (which also bounds the speculation width). None of them changes a result (every setting computes the same
 // greedy clustering), so the tuner times whole clusterings of the caller's
 // store and keeps the fastest setting; it also checks that every trial
 // returned the same clusters. No counterpart in the reference, whose launch
@@ -25,7 +26,7 @@ struct Knob {
 }  // namespace
 
 int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, gsdr_tuning* out) {
-  if (max_trials <= 0) max_trials = 16;
+  if (max_trials <= 0) max_trials = 20;
   gsdr_params p = p0;
   p.mode = GSDR_MODE_BATCHED;
   p.timing = 0;
@@ -61,6 +62,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
   double ms_best = ms_default;
   int trials = 1;
   const Knob knobs[] = {
+      {&Tuning::list_factor, {32}},
       {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
       {&Tuning::spec_max, {16, 32}},
       {&Tuning::align_blocks, {2, 8, 0}},
@@ -86,7 +88,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
   if (trials < max_trials &&
       (best.swap_ratio != Tuning().swap_ratio || best.spec_max != Tuning().spec_max ||
        best.align_blocks != Tuning().align_blocks || best.filter_blocks != Tuning().filter_blocks ||
-       best.fork_shapes != Tuning().fork_shapes)) {
+       best.fork_shapes != Tuning().fork_shapes || best.list_factor != Tuning().list_factor)) {
     double again = 0;
     if ((st = run(best, &again, false))) return finish(st);
     ++trials;
@@ -106,6 +108,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
     out->filter_blocks = best.filter_blocks;
     out->align_blocks = best.align_blocks;
     out->fork_shapes = best.fork_shapes;
+    out->list_factor = best.list_factor;
     out->trials = trials;
     out->ms_default = ms_default;
     out->ms_tuned = ms_best;

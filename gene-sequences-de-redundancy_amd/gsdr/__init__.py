@@ -75,12 +75,14 @@ class Tuning(C.Structure):
         ("align_blocks", C.c_int32),
         ("trials", C.c_int32),
         ("fork_shapes", C.c_int32),
+        ("list_factor", C.c_int32),
+        ("reserved2", C.c_int32),
         ("ms_default", C.c_double),
         ("ms_tuned", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved2"}
 
 
 # every entry point of include/gsdr.h: (name, restype, argtypes)

@@ -93,6 +93,9 @@ typedef struct gsdr_tuning {
                              shape kernel's resident blocks per CU) */
   int32_t trials;         /* autotune: clusterings timed */
   int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
+  int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
+                             sequence (default 8); it also bounds k_spec * candidates */
+  int32_t reserved2;
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
 } gsdr_tuning;
@@ -164,7 +167,7 @@ int gsdr_cluster_jobs(gsdr_ctx *ctx, int32_t njobs, gsdr_store *const *stores,
                       const gsdr_params *params, int32_t *const *cluster_out, gsdr_stats *stats);
 /* Autotuner (no counterpart in the reference): clusters `store` with params
  * (forced to the batched form) under coordinate descent over the knobs of
- * gsdr_tuning, at most max_trials clusterings (0 = 16), and keeps the fastest
+ * gsdr_tuning, at most max_trials clusterings (0 = 20), and keeps the fastest
  * set in the context; later calls with params.tune = 1 use it. Every trial
  * must return the same clusters, else GSDR_E_DEVICE. out (may be NULL) gets
  * the chosen knobs and timings. */

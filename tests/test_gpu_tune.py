@@ -23,9 +23,11 @@ def test_every_knob_setting_gives_the_oracle_clusters(engine, tmp_path, law, n, 
     r, ocl = _oracle(tmp_path, gsdr.Records.generate(law, 1000 + law, n), t)
     store = engine.load(r, gsdr.auto_word_length(t))
     try:
-        settings = [(2, 64, 8, 4, 1), (0, 64, 8, 0, 0), (8, 1, 1, 1, 1), (3, 16, 16, 2, 0), (1, 7, 2, 8, 1)]
-        for swap, spec, fb, ab, fork in settings:
-            tu = gsdr.Tuning(swap_ratio=swap, spec_max=spec, filter_blocks=fb, align_blocks=ab, fork_shapes=fork)
+        settings = [(2, 64, 8, 4, 1, 8), (0, 64, 8, 0, 0, 64), (8, 1, 1, 1, 1, 1), (3, 16, 16, 2, 0, 32),
+                    (1, 7, 2, 8, 1, 2)]
+        for swap, spec, fb, ab, fork, lf in settings:
+            tu = gsdr.Tuning(swap_ratio=swap, spec_max=spec, filter_blocks=fb, align_blocks=ab, fork_shapes=fork,
+                             list_factor=lf)
             engine.set_tuning(tu)
             got = engine.tuning()
             assert (got.swap_ratio, got.spec_max, got.filter_blocks, got.align_blocks, got.fork_shapes) == (
@@ -55,16 +57,16 @@ def test_autotune_then_tuned_calls_match(engine, tmp_path):
         engine.set_tuning(None)
         store.free()
     d = engine.tuning()
-    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes) == (4, 64, 8, 4, 1)
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, 8, 4, 1, 8)
 
 
 def test_tuning_out_of_range_is_rejected(engine):
     for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0),
-                dict(align_blocks=65), dict(align_blocks=-1), dict(fork_shapes=2)]:
-        tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1)
+                dict(align_blocks=65), dict(align_blocks=-1), dict(fork_shapes=2), dict(list_factor=0)]:
+        tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1, list_factor=8)
         for k, v in bad.items():
             setattr(tu, k, v)
         with pytest.raises(gsdr.GsdrError):
             engine.set_tuning(tu)
     d = engine.tuning()
-    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes) == (4, 64, 8, 4, 1)
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, 8, 4, 1, 8)
