@@ -37,7 +37,9 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
   if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
-  return launch_resolve(L.ra, q);
+  if ((e = launch_resolve(L.ra, q)) != hipSuccess) return e;
+  // with several jobs, bound the next step's widths by the lists' capacity
+  return L.ra.njobs > 1 ? launch_clamp_spec(L.ra.jobs, L.ra.njobs, L.ra.pair_budget, q) : hipSuccess;
 }
 
 StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, int32_t* jobs_left, const Tuning& T) {
@@ -157,7 +159,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
     while (true) {
       HIP_TRY(launch_step(c, L, q, ev), "step launch");
-      launches += 3 + 2 * (L.smax + 1) + 1;
+      launches += 3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0);
       ++steps_launched;
       HIP_TRY(hipMemcpyAsync(c->h_state, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
       HIP_TRY(hipStreamSynchronize(q), "step sync");
@@ -197,7 +199,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       if (e == hipSuccess) e = hipStreamSynchronize(q);
       if (e != hipSuccess) { rc = hip_fail(e, "graph replay"); break; }
       steps_launched += B;
-      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1);
+      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0));
       if (*h_left == 0) break;
       if (steps_launched > step_limit + B) { rc = fail(GSDR_E_DEVICE, "steps did not converge"); break; }
     }

@@ -78,6 +78,7 @@ int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs) {
   w.item_cap = cap;
   w.jobs_cap = std::max(njobs, 1);
   w.bytes = F;
+  w.generation = ++c->ws_generations;
   if (c->h_state_cap < w.jobs_cap) {
     if (c->h_state) (void)hipHostFree(c->h_state);
     HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * w.jobs_cap, hipHostMallocDefault), "hipHostMalloc");

@@ -26,11 +26,11 @@ struct Workspace {
   int32_t jobs_cap = 0;
   int32_t* jobs_left = nullptr;
   int64_t bytes = 0;
+  int64_t generation = 0;          // changes with every (re)allocation
 };
 
-// Launch knobs that change speed, never results: the role-swap ratio, the
-// cap on speculative representatives per step and the filter / align
-// workgroups per CU. gsdr_autotune measures them on a store.
+// Launch knobs that change speed, never results (gsdr_tuning in the C ABI);
+// gsdr_autotune measures them on a store.
 struct Tuning {
   int swap_ratio = gsdr::kSwapRatio;
   int spec_max = gsdr::kMaxSpec;
@@ -43,6 +43,7 @@ struct Tuning {
 struct gsdr_ctx {
   int device = 0;
   Tuning tuned;                       // used by calls with params.tune = 1
+  int64_t ws_generations = 0;         // workspace allocations so far
   bool has_tuning = false;
   hipStream_t stream = nullptr;       // kernels
   hipStream_t upload = nullptr;       // store uploads (may overlap kernels of another job)
@@ -122,10 +123,9 @@ int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs);
 void workspace_release(gsdr_ctx* c);
 int max_shape_for(int n);
 struct AlignArgs;
-// Launches the alignment kernels of shapes 0..smax concurrently on the side
-// streams, forked from and joined back into `main`.
-// fork = 1: the shape kernels run concurrently on the side streams (fork /
-// join events); 0: one after another on `main` (no cross-queue waits).
+// Launches the alignment kernels of shapes 0..smax. fork = 1: concurrently
+// on the side streams, forked from and joined back into `main`; 0: one after
+// another on `main` (no cross-queue waits).
 hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main,
                                int fork = 1);
 

@@ -129,9 +129,10 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
         k2 = want >= up ? up : (want <= down ? down : (int)want);
       }
     }
-    // every survivor of a step may land in one shape list: keep k_spec * nc
-    // of all jobs within its capacity
-    const long long cap = a.pair_budget / ((long long)a.njobs * (long long)max(nc2, 1));
+    // every survivor of a step may land in one shape list: k_spec * nc of
+    // all jobs must stay within its capacity (this job alone here; the
+    // clamp that follows the step bounds the sum over the jobs)
+    const long long cap = a.pair_budget / (long long)max(nc2, 1);
     if ((long long)k2 > cap) k2 = (int)cap;
     if (k2 > nc2) k2 = nc2;
     if (k2 < 1) k2 = 1;
@@ -149,27 +150,41 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
 }
 
 // Bounds k_spec of every running job so that k_spec * nc summed over the
-// njobs jobs fits the alignment lists (needed when jobs join a table whose
-// k_spec values were chosen for fewer jobs). Any k_spec is exact.
-__global__ void clamp_spec_kernel(const JobDev* __restrict__ jobs, int njobs, long long pair_budget) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= njobs) return;
-  JobState* S = jobs[j].state;
-  if (S->done) return;
-  const long long cap = pair_budget / ((long long)njobs * (long long)max(S->nc, 1));
-  int k = S->k_spec;
-  if ((long long)k > cap) k = (int)max(cap, 1ll);
-  if (k > S->nc) k = S->nc;
-  if (k < 1) k = 1;
-  if (k != S->k_spec) {
-    S->k_spec = k;
-    S->tmask = 1ull;
-    S->kt = 1;
+// jobs fits the alignment lists: with T the total of nc over the running
+// jobs, k_spec <= pair_budget / T gives sum(k_spec * nc) <= pair_budget.
+// Runs after every step's resolve (whose K choice ignores the other jobs)
+// and when jobs join a table. One block; any k_spec is exact.
+__global__ __launch_bounds__(1024) void clamp_spec_kernel(const JobDev* __restrict__ jobs, int njobs,
+                                                          long long pair_budget) {
+  __shared__ int s_part[16];
+  long long tot = 0;
+  for (int base = 0; base < njobs; base += 1024) {
+    const int j = base + threadIdx.x;
+    int v = 0;
+    if (j < njobs) {
+      const JobState* S = jobs[j].state;
+      if (!S->done) v = S->nc;
+    }
+    tot += block_sum<1024>(v, s_part);
+  }
+  const long long cap = pair_budget / (tot > 0 ? tot : 1ll);
+  for (int j = threadIdx.x; j < njobs; j += 1024) {
+    JobState* S = jobs[j].state;
+    if (S->done) continue;
+    int k = S->k_spec;
+    if ((long long)k > cap) k = (int)max(cap, 1ll);
+    if (k > S->nc) k = S->nc;
+    if (k < 1) k = 1;
+    if (k != S->k_spec) {
+      S->k_spec = k;
+      S->tmask = 1ull;
+      S->kt = 1;
+    }
   }
 }
 
 hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s) {
-  hipLaunchKernelGGL(clamp_spec_kernel, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs, njobs, pair_budget);
+  hipLaunchKernelGGL(clamp_spec_kernel, dim3(1), dim3(1024), 0, s, jobs, njobs, pair_budget);
   return hipGetLastError();
 }
 

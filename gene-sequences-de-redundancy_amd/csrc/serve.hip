@@ -49,7 +49,39 @@ struct gsdr_server {
   std::vector<ServeJob*> waiting;  // loaded, not yet in the device table
   std::vector<ServeJob*> running;  // in the device table
   int steps_per_call = 4;
+  // the steps of one call, captured for the current job table and replayed
+  // while the table stays the same
+  hipGraphExec_t exec = nullptr;
+  std::vector<ServeJob*> graph_jobs;
+  int64_t graph_ws = 0;  // workspace generation the graph's pointers belong to
 };
+
+static void drop_graph(gsdr_server* srv) {
+  if (srv->exec) (void)hipGraphExecDestroy(srv->exec);
+  srv->exec = nullptr;
+  srv->graph_jobs.clear();
+}
+
+// Captures steps_per_call steps over the current table into srv->exec.
+static hipError_t capture_steps(gsdr_server* srv, const StepLaunch& L, hipStream_t q) {
+  gsdr_ctx* c = srv->c;
+  drop_graph(srv);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int s = 0; s < srv->steps_per_call && e == hipSuccess; ++s) e = launch_step(c, L, q, nullptr);
+  hipError_t e2 = hipStreamEndCapture(q, &graph);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&srv->exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    srv->exec = nullptr;
+    return e;
+  }
+  srv->graph_jobs = srv->running;
+  srv->graph_ws = c->ws.generation;
+  return hipSuccess;
+}
 
 // Upper bound of a job's HBM use: packed store, k-mer profile (at most one
 // entry per base), per-sequence arrays and round state.
@@ -67,9 +99,10 @@ int gsdr_serve_open(gsdr_ctx* c, int64_t hbm_budget, int32_t max_slots, gsdr_ser
   s->c = c;
   s->budget = hbm_budget;
   s->max_slots = max_slots;
-  // alignment lists shared by all running jobs; resolve keeps k_spec * nc of
-  // every job within them
-  int st = workspace_reserve(c, 8ll << 20, max_slots);
+  // alignment lists shared by all running jobs (32 M items per tile shape,
+  // 6 GiB); the clamp after each step keeps k_spec * nc of all jobs within
+  // them
+  int st = workspace_reserve(c, 32ll << 20, max_slots);
   if (st) return st;
   hipError_t e;
   if ((e = hipMemsetAsync(c->ws.counters, 0, 2 * kNumShapes * sizeof(int32_t), c->stream)) != hipSuccess ||
@@ -168,9 +201,13 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
         return hip_fail(e, "init");
     }
     if ((e = launch_clamp_spec(w.jobs, nj, w.item_cap, q)) != hipSuccess) return hip_fail(e, "clamp");
-    const StepLaunch L = make_step_launch(c, nj, smax, kmask, nullptr);
-    for (int s = 0; s < srv->steps_per_call; ++s)
-      if ((e = launch_step(c, L, q, nullptr)) != hipSuccess) return hip_fail(e, "serve step");
+    // recapture when the table changed, or when another call on the context
+    // reallocated the workspace the graph's kernels point into
+    if (!srv->exec || srv->graph_jobs != srv->running || srv->graph_ws != c->ws.generation) {
+      const StepLaunch L = make_step_launch(c, nj, smax, kmask, nullptr);
+      if ((e = capture_steps(srv, L, q)) != hipSuccess) return hip_fail(e, "serve capture");
+    }
+    if ((e = hipGraphLaunch(srv->exec, q)) != hipSuccess) return hip_fail(e, "serve steps");
     std::vector<JobState> hs((size_t)nj);
     for (int i = 0; i < nj; ++i)
       if ((e = hipMemcpyAsync(&hs[i], table[i].state, sizeof(JobState), hipMemcpyDeviceToHost, q)) != hipSuccess)
@@ -237,6 +274,7 @@ int gsdr_serve_poll(gsdr_server* srv, int64_t job_id, int32_t* cluster_out, gsdr
 
 void gsdr_serve_close(gsdr_server* srv) {
   if (!srv) return;
+  drop_graph(srv);
   for (auto& kv : srv->jobs)
     if (kv.second->store) store_release(*kv.second->store);
   delete srv;

@@ -56,6 +56,21 @@ def test_drop_mode_matches_oracle(engine, tmp_path, config, n, t):
         assert st.reps == ost["reps"]
 
 
+@pytest.mark.parametrize("config,n,t,k", [(1, 2500, 0.95, 8), (2, 2000, 0.99, 7), (4, 3000, 0.97, 0),
+                                          (1, 2000, 0.90, 7)])
+def test_long_words_match_oracle(engine, tmp_path, config, n, t, k):
+    """Word lengths 7 and 8 keep their shared-word tables in global memory,
+    one per representative slot of a step; both forms against the oracle."""
+    recs = gsdr.Records.generate(config, 2000 + config, n)
+    for mode in (gsdr.MODE_PLAIN, gsdr.MODE_BATCHED):
+        cl, ocl, st, ost, same_file = run_both(engine, tmp_path, recs, t, mode=mode, k=k)
+        bad = np.nonzero(cl != ocl)[0]
+        assert len(bad) == 0, f"{len(bad)} assignments differ, first at {bad[:5]}"
+        assert same_file
+        assert st.pairs == st.rej_length + st.rej_comp + st.rej_word + st.aligned
+        assert st.reps == ost["reps"]
+
+
 def test_iupac_crlf_edge_set(engine, tmp_path):
     rng = np.random.default_rng(5)
     lines = []
