@@ -37,7 +37,7 @@ struct Tuning {
   int filter_blocks = 8;
   int align_blocks = 4;  // 0: each shape kernel's resident blocks per CU
   int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
-  int list_factor = 8;   // alignment list capacity per shape: list_factor items per sequence
+  int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
 };
 
 struct gsdr_ctx {

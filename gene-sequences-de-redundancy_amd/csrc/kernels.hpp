@@ -83,6 +83,10 @@ struct StoreView {
 constexpr int kMaxSpec = 64;
 // Candidates per filter work unit (one per 16-lane group of a 256-thread block).
 constexpr int kFilterChunk = 16;
+// With one table per block (long words) a unit takes four candidates per
+// 16-lane group, whose record loads are issued together.
+__host__ __device__ constexpr int filter_subs(int group) { return group == 1 ? 4 : 1; }
+__host__ __device__ constexpr int filter_chunk(int group) { return kFilterChunk * filter_subs(group); }
 
 // Per-job round state on the device. The candidate list cand[cur][0..nc)
 // holds the unclustered sorted indices in ascending order; its first k_spec
@@ -107,16 +111,16 @@ struct JobState {
 enum FilterPhase { PHASE_REPS = 0, PHASE_MAIN = 1 };
 
 // Work units of a job in a phase: (group of up to `group` representative
-// slots, kFilterChunk candidate positions). Phase A: slots 0..K-1 against
+// slots, filter_chunk(group) candidate positions). Phase A: slots 0..K-1 against
 // positions 0..K-1; phase B: the kt true representatives against positions
 // K..nc-1.
 __host__ __device__ inline int phase_units(const JobState& s, int phase, int group) {
   if (s.done) return 0;
   const int K = s.k_spec;
   if (phase == PHASE_REPS)
-    return K > 1 ? ((K + group - 1) / group) * ((K + kFilterChunk - 1) / kFilterChunk) : 0;
+    return K > 1 ? ((K + group - 1) / group) * ((K + filter_chunk(group) - 1) / filter_chunk(group)) : 0;
   const int rest = s.nc - K;
-  return rest > 0 ? ((s.kt + group - 1) / group) * ((rest + kFilterChunk - 1) / kFilterChunk) : 0;
+  return rest > 0 ? ((s.kt + group - 1) / group) * ((rest + filter_chunk(group) - 1) / filter_chunk(group)) : 0;
 }
 
 // Representatives whose shared-word tables a filter block holds at once:

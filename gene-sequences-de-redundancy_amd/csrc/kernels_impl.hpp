@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     const int p0 = a.phase == PHASE_REPS ? 0 : K;
     const int p1 = a.phase == PHASE_REPS ? K : nc;
     const int members = a.phase == PHASE_REPS ? K : S->kt;
-    const int chunks = (p1 - p0 + kFilterChunk - 1) / kFilterChunk;
+    const int chunks = (p1 - p0 + filter_chunk(G) - 1) / filter_chunk(G);
     const int lu = u - s_pre[job];
     const int gi = lu / chunks;
     const int chunk = lu - gi * chunks;
@@ -200,131 +200,149 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       cg = gi;
     }
     // every lane of the block runs the whole unit (no early exit): the
-    // survivor append below is a wave-wide operation
-    const int pos = p0 + chunk * kFilterChunk + grp;
-    const bool inpos = pos < p1;
+    // survivor append below is a wave-wide operation. The NS candidates of a
+    // group have their index and record loads issued together.
+    constexpr int NS = filter_subs(G);
     const bool drop = (J.flags & kJobDropFilters) != 0;
-    const int q = inpos ? cand[pos] : 0;
-    const int mq = J.st.m[q];
-    const int bcq = J.st.bc[q];
-    const int c = bcq + 1;
-    const uint4 cq = J.st.comp[q];
-    if (gl == 0 && inpos) bytes += 28;  // id, m, bc, comp
-    // length and composition per representative of the group
-    uint32_t mask = 0;
+    int qv[NS], mqv[NS], bcv[NS];
+    uint4 cqv[NS];
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (!inpos || g >= gcount || pos <= s_slot[g]) continue;
-      if (c > min(s_n[g], mq)) {
-        if (gl == 0) ++rej_len;
-        continue;
-      }
-      const uint4 cr = s_comp[g];
-      const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
-      if (!drop && sum <= bcq) {
-        if (gl == 0) ++rej_comp;
-        continue;
-      }
-      mask |= 1u << g;
+    for (int sub = 0; sub < NS; ++sub) {
+      const int ps = p0 + chunk * filter_chunk(G) + sub * kFilterChunk + grp;
+      qv[sub] = ps < p1 ? cand[ps] : 0;
     }
-    // shared words: q's profile read once, one table lookup per survivor so
-    // far (none when no representative is left; drop mode reads no profile
-    // and passes every pair)
-    const int nw = 1 << (2 * J.k);
-    const int64_t e0 = J.st.kpoff[q], e1 = (drop || mask == 0) ? e0 : J.st.kpoff[q + 1];
-    int acc[G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g] = 0;
-    // the profile loads are latency bound: NB (16, or 8 beside 16 tables,
-    // which need the registers), then 4, in flight per lane
-    constexpr int NB = G >= 16 ? 8 : 16;
-    int64_t e = e0 + gl;
-    for (; e + (NB - 1) * 16 < e1; e += NB * 16) {
-      uint32_t xs[NB];
+    for (int sub = 0; sub < NS; ++sub) {
+      mqv[sub] = J.st.m[qv[sub]];
+      bcv[sub] = J.st.bc[qv[sub]];
+      cqv[sub] = J.st.comp[qv[sub]];
+    }
 #pragma unroll
-      for (int i = 0; i < NB; ++i) xs[i] = J.st.kp[e + 16 * i];
+    for (int sub = 0; sub < NS; ++sub) {
+      const int pos = p0 + chunk * filter_chunk(G) + sub * kFilterChunk + grp;
+      const bool inpos = pos < p1;
+      const int q = qv[sub];
+      const int mq = mqv[sub];
+      const int bcq = bcv[sub];
+      const int c = bcq + 1;
+      const uint4 cq = cqv[sub];
+      if (gl == 0 && inpos) bytes += 28;  // id, m, bc, comp
+      // length and composition per representative of the group
+      uint32_t mask = 0;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        if (!((mask >> g) & 1u)) continue;
-        const uint16_t* tg = table + g * nw;
-        int sg = 0;
-#pragma unroll
-        for (int i = 0; i < NB; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
-        acc[g] += sg;
-      }
-    }
-    for (; e + 48 < e1; e += 64) {
-      const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (!((mask >> g) & 1u)) continue;
-        const uint16_t* tg = table + g * nw;
-        acc[g] += min((uint32_t)tg[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)tg[x1 & 0xFFFFu], x1 >> 16) +
-                  min((uint32_t)tg[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)tg[x3 & 0xFFFFu], x3 >> 16);
-      }
-    }
-    for (; e < e1; e += 16) {
-      const uint32_t x = J.st.kp[e];
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (!((mask >> g) & 1u)) continue;
-        acc[g] += min((uint32_t)table[g * nw + (x & 0xFFFFu)], x >> 16);
-      }
-    }
-    if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
-    const int wcq = drop ? 0 : J.st.wc[q];
-    // per representative g of the group: the 16-lane total, kept by lane g
-    int mine = 0;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (!((mask >> g) & 1u)) continue;
-      int sg = acc[g];
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) sg += __shfl_xor(sg, off, 16);
-      if (gl == g) mine = sg;
-    }
-    // lane g decides representative g; survivors of the wave are appended
-    // with one list reservation per tile shape present
-    const bool live = gl < G && ((mask >> gl) & 1u);
-    const bool pass = live && mine >= wcq;
-    if (live && !pass) ++rej_word;
-    int sh = 0;
-    AlignItem it{0, 0, 0, 0};
-    if (pass) {
-      const int r = s_rep[gl], slot = s_slot[gl], n = s_n[gl];
-      ++passed;
-      bytes += 16;
-      cband += (unsigned long long)band_cells(n, mq, c);
-      cfull += (unsigned long long)n * (unsigned long long)mq;
-      const bool sw = swap_roles(n, mq, a.swap_ratio);
-      sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
-      it.text = sw ? q : r;
-      it.query = sw ? r : q;
-      it.job = job;
-      it.slot = slot | (sw ? kSwapFlag : 0);
-    }
-    const int wl = tid & 63;
-    bool todo = pass;
-    for (uint64_t pend = __builtin_amdgcn_ballot_w64(todo); pend != 0;) {
-      const int l = (int)__builtin_ctzll(pend);
-      const int s0 = __shfl(sh, l);
-      const uint64_t same = __builtin_amdgcn_ballot_w64(todo && sh == s0);
-      int base = 0;
-      if (wl == l) base = atomicAdd(&a.bin_count[s0], (int)__popcll(same));
-      base = __shfl(base, l);
-      if (todo && sh == s0) {
-        const int at = base + (int)__popcll(same & ((1ull << wl) - 1ull));
-        if (at < a.item_cap) {
-          a.items[(int64_t)sh * a.item_cap + at] = it;
-        } else {
-          // the list is full: count it so the host fails the call (the
-          // host bounds k_spec * nc so that this does not happen)
-          atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
-          atomicSub(&a.bin_count[sh], 1);
+        if (!inpos || g >= gcount || pos <= s_slot[g]) continue;
+        if (c > min(s_n[g], mq)) {
+          if (gl == 0) ++rej_len;
+          continue;
         }
-        todo = false;
+        const uint4 cr = s_comp[g];
+        const int sum = (int)(min(cr.x, cq.x) + min(cr.y, cq.y) + min(cr.z, cq.z) + min(cr.w, cq.w));
+        if (!drop && sum <= bcq) {
+          if (gl == 0) ++rej_comp;
+          continue;
+        }
+        mask |= 1u << g;
       }
-      pend &= ~same;
+      // shared words: q's profile read once, one table lookup per survivor so
+      // far (none when no representative is left; drop mode reads no profile
+      // and passes every pair)
+      const int nw = 1 << (2 * J.k);
+      const int64_t e0 = J.st.kpoff[q], e1 = (drop || mask == 0) ? e0 : J.st.kpoff[q + 1];
+      int acc[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) acc[g] = 0;
+      // the profile loads are latency bound: NB (16, or 8 beside 16 tables,
+      // which need the registers), then 4, in flight per lane
+      constexpr int NB = G >= 16 ? 8 : 16;
+      int64_t e = e0 + gl;
+      for (; e + (NB - 1) * 16 < e1; e += NB * 16) {
+        uint32_t xs[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) xs[i] = J.st.kp[e + 16 * i];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (!((mask >> g) & 1u)) continue;
+          const uint16_t* tg = table + g * nw;
+          int sg = 0;
+#pragma unroll
+          for (int i = 0; i < NB; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
+          acc[g] += sg;
+        }
+      }
+      for (; e + 48 < e1; e += 64) {
+        const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (!((mask >> g) & 1u)) continue;
+          const uint16_t* tg = table + g * nw;
+          acc[g] += min((uint32_t)tg[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)tg[x1 & 0xFFFFu], x1 >> 16) +
+                    min((uint32_t)tg[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)tg[x3 & 0xFFFFu], x3 >> 16);
+        }
+      }
+      for (; e < e1; e += 16) {
+        const uint32_t x = J.st.kp[e];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (!((mask >> g) & 1u)) continue;
+          acc[g] += min((uint32_t)table[g * nw + (x & 0xFFFFu)], x >> 16);
+        }
+      }
+      if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
+      const int wcq = drop ? 0 : J.st.wc[q];
+      // per representative g of the group: the 16-lane total, kept by lane g
+      int mine = 0;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (!((mask >> g) & 1u)) continue;
+        int sg = acc[g];
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) sg += __shfl_xor(sg, off, 16);
+        if (gl == g) mine = sg;
+      }
+      // lane g decides representative g; survivors of the wave are appended
+      // with one list reservation per tile shape present
+      const bool live = gl < G && ((mask >> gl) & 1u);
+      const bool pass = live && mine >= wcq;
+      if (live && !pass) ++rej_word;
+      int sh = 0;
+      AlignItem it{0, 0, 0, 0};
+      if (pass) {
+        const int r = s_rep[gl], slot = s_slot[gl], n = s_n[gl];
+        ++passed;
+        bytes += 16;
+        cband += (unsigned long long)band_cells(n, mq, c);
+        cfull += (unsigned long long)n * (unsigned long long)mq;
+        const bool sw = swap_roles(n, mq, a.swap_ratio);
+        sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
+        it.text = sw ? q : r;
+        it.query = sw ? r : q;
+        it.job = job;
+        it.slot = slot | (sw ? kSwapFlag : 0);
+      }
+      const int wl = tid & 63;
+      bool todo = pass;
+      for (uint64_t pend = __builtin_amdgcn_ballot_w64(todo); pend != 0;) {
+        const int l = (int)__builtin_ctzll(pend);
+        const int s0 = __shfl(sh, l);
+        const uint64_t same = __builtin_amdgcn_ballot_w64(todo && sh == s0);
+        int base = 0;
+        if (wl == l) base = atomicAdd(&a.bin_count[s0], (int)__popcll(same));
+        base = __shfl(base, l);
+        if (todo && sh == s0) {
+          const int at = base + (int)__popcll(same & ((1ull << wl) - 1ull));
+          if (at < a.item_cap) {
+            a.items[(int64_t)sh * a.item_cap + at] = it;
+          } else {
+            // the list is full: count it so the host fails the call (the
+            // host bounds k_spec * nc so that this does not happen)
+            atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
+            atomicSub(&a.bin_count[sh], 1);
+          }
+          todo = false;
+        }
+        pend &= ~same;
+      }
     }
   }
   unsigned long long v[7] = {rej_len, rej_comp, rej_word, passed, bytes, cband, cfull};

@@ -62,7 +62,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
   double ms_best = ms_default;
   int trials = 1;
   const Knob knobs[] = {
-      {&Tuning::list_factor, {32}},
+      {&Tuning::list_factor, {8, 16}},
       {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
       {&Tuning::spec_max, {16, 32}},
       {&Tuning::align_blocks, {2, 8, 0}},

@@ -94,7 +94,7 @@ typedef struct gsdr_tuning {
   int32_t trials;         /* autotune: clusterings timed */
   int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
   int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
-                             sequence (default 8); it also bounds k_spec * candidates */
+                             sequence (default 64); it also bounds k_spec * candidates */
   int32_t reserved2;
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
