@@ -999,7 +999,13 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     int res = 0;
     if (c <= min(n, mq)) {
       const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1};
+      // a step ends when its longest pair does: waves on long pairs get the
+      // SIMD's issue slots first, the short pairs fill the gaps
+      if (mq >= 16384) __builtin_amdgcn_s_setprio(3);
+      else if (mq >= 4096) __builtin_amdgcn_s_setprio(2);
+      else if (mq >= 1024) __builtin_amdgcn_s_setprio(1);
       res = decide64<W>(P, X, lane, cells, cdec);
+      __builtin_amdgcn_s_setprio(0);
     }
     if (lane == 0) {
       if (a.result) {
