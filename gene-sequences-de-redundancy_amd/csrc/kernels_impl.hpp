@@ -618,6 +618,8 @@ __device__ inline void align_loop(const AlignArgs& a) {
 
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
+    // one pair per wave: whether it is live is one scalar for the block
+    const bool wave_live = S == 64 && __builtin_amdgcn_ballot_w64(active) != 0;
     for (int r0 = 0; r0 < 32; r0 += UR) {
       const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
 #pragma unroll
@@ -643,7 +645,15 @@ __device__ inline void align_loop(const AlignArgs& a) {
         // lane). Rows past a query end and segments without a pair change
         // nothing, so they do not count.
         uint32_t cin;
-        if (__builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0) {
+        bool common;
+        if constexpr (S == 64) {
+          // wave_live and qn are scalars here (qn comes from the scalar
+          // query word): only pall needs a ballot
+          common = !(wave_live && qn == 0u) || __builtin_amdgcn_ballot_w64(pall == ~0u) == 0;
+        } else {
+          common = __builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0;
+        }
+        if (common) {
           // lane 0 of the wave keeps the old value 0; a segment's first lane
           // of a narrower tile is cleared
           cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
