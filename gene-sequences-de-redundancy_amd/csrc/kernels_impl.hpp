@@ -479,6 +479,9 @@ __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W],
 // each block end the zero count z gives the exact early decisions z >= c
 // (accept) and z + rows_left < c (reject); the window slides by one lane when
 // the band's left edge has passed the lowest lane.
+struct RaggedRows { static constexpr bool value = true; };
+struct FullRows { static constexpr bool value = false; };
+
 template <int S, int W>
 __device__ inline void align_loop(const AlignArgs& a) {
   constexpr uint64_t KTOP = top_lanes<S>();
@@ -618,63 +621,73 @@ __device__ inline void align_loop(const AlignArgs& a) {
 
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
     constexpr int UR = W <= 4 ? 32 : 8;
-    // one pair per wave: whether it is live is one scalar for the block
-    const bool wave_live = S == 64 && __builtin_amdgcn_ballot_w64(active) != 0;
-    for (int r0 = 0; r0 < 32; r0 += UR) {
-      const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
+    // Segments live in this block, and whether any of them ends its query
+    // inside it. Without such a segment every live row of the block is a
+    // real row, so the carry shortcut below needs only the ballot of
+    // all-ones sums restricted to the live segments (a scalar mask).
+    const uint64_t live = __builtin_amdgcn_ballot_w64(active);
+    const bool ragged = __builtin_amdgcn_ballot_w64(active && qnv != 0u) != 0;
+    auto run_rows = [&](auto tag) {
+      constexpr bool kRagged = decltype(tag)::value;
+      for (int r0 = 0; r0 < 32; r0 += UR) {
+        const uint32_t qlc = qlo >> r0, qhc = qhi >> r0, qnc = qnv >> r0;
 #pragma unroll
-      for (int r = 0; r < UR; ++r) {
-        // bit r as an all-zeros / all-ones mask (one signed bit-field extract)
-        const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
-        const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
-        const uint32_t qn = (uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
-        uint32_t x[W], s[W];
-        uint32_t carry = 0u, pall = ~0u;
+        for (int r = 0; r < UR; ++r) {
+          // bit r as an all-zeros / all-ones mask (one signed bit-field extract)
+          const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
+          const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
+          const uint32_t qn = (uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
+          uint32_t x[W], s[W];
+          uint32_t carry = 0u, pall = ~0u;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          x[w] = (tl[w] ^ ql) | (th[w] ^ qh) | qn;
-          const uint32_t u = v[w] & ~x[w];
-          uint32_t co;
-          s[w] = __builtin_addc(v[w], u, carry, &co);
-          carry = co;
-          pall &= s[w];
-        }
-        // A lane passes a carry on only when its whole sum is ones. When no
-        // lane of a live row has that, the carry into lane l is the carry
-        // out of lane l-1 (one lane shift; none into a segment's first
-        // lane). Rows past a query end and segments without a pair change
-        // nothing, so they do not count.
-        uint32_t cin;
-        bool common;
-        if constexpr (S == 64) {
-          // wave_live and qn are scalars here (qn comes from the scalar
-          // query word): only pall needs a ballot
-          common = !(wave_live && qn == 0u) || __builtin_amdgcn_ballot_w64(pall == ~0u) == 0;
-        } else {
-          common = __builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0;
-        }
-        if (common) {
-          // lane 0 of the wave keeps the old value 0; a segment's first lane
-          // of a narrower tile is cleared
-          cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-          if constexpr (S < 64) {
-            if (sl == 0) cin = 0u;
+          for (int w = 0; w < W; ++w) {
+            x[w] = (tl[w] ^ ql) | (th[w] ^ qh) | qn;
+            const uint32_t u = v[w] & ~x[w];
+            uint32_t co;
+            s[w] = __builtin_addc(v[w], u, carry, &co);
+            carry = co;
+            pall &= s[w];
           }
-        } else {
-          const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
-          const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
-          const uint64_t X = G | P;
-          const uint64_t C = (X + G) ^ X ^ G;
-          cin = (uint32_t)(C >> lane) & 1u;
-        }
+          // A lane passes a carry on only when its whole sum is ones. When
+          // no lane of a live row has that, the carry into lane l is the
+          // carry out of lane l-1 (one lane shift; none into a segment's
+          // first lane). Rows past a query end and segments without a pair
+          // change nothing, so they do not count.
+          uint32_t cin;
+          bool common;
+          if constexpr (kRagged) {
+            common = __builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0;
+          } else {
+            common = (__builtin_amdgcn_ballot_w64(pall == ~0u) & live) == 0;
+          }
+          if (common) {
+            // lane 0 of the wave keeps the old value 0; a segment's first
+            // lane of a narrower tile is cleared
+            cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+            if constexpr (S < 64) {
+              if (sl == 0) cin = 0u;
+            }
+          } else {
+            const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
+            const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
+            const uint64_t X = G | P;
+            const uint64_t C = (X + G) ^ X ^ G;
+            cin = (uint32_t)(C >> lane) & 1u;
+          }
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-          uint32_t co;
-          const uint32_t t = __builtin_addc(s[w], 0u, cin, &co);
-          cin = co;
-          v[w] = t | (v[w] & x[w]);
+          for (int w = 0; w < W; ++w) {
+            uint32_t co;
+            const uint32_t t = __builtin_addc(s[w], 0u, cin, &co);
+            cin = co;
+            v[w] = t | (v[w] & x[w]);
+          }
         }
       }
+    };
+    if (ragged) {
+      run_rows(RaggedRows{});
+    } else {
+      run_rows(FullRows{});
     }
 
     // ---- block end: exact early decisions on the zero counts of all
