@@ -636,7 +636,8 @@ __device__ inline void align_loop(const AlignArgs& a) {
           // bit r as an all-zeros / all-ones mask (one signed bit-field extract)
           const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
           const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
-          const uint32_t qn = (uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
+          // (without a query end in the block, live rows have qn = 0)
+          const uint32_t qn = kRagged ? (uint32_t)((int32_t)(qnc << (31 - r)) >> 31) : 0u;
           uint32_t x[W], s[W];
           uint32_t carry = 0u, pall = ~0u;
 #pragma unroll
