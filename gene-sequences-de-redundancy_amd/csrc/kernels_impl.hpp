@@ -426,17 +426,19 @@ __device__ __forceinline__ void row_sum_masks(const uint32_t (&v)[W], const uint
 // lanes, then V' = (S + carry in) | (V ^ U). A lane passes a carry in on only
 // when all of its words of S are ones, which needs its first word to be all
 // ones. When no lane has that, the carry into lane l is simply the carry out
-// of lane l-1 (one lane shift of the wave, DPP wave_shr:1); otherwise the full
-// generate/propagate look-ahead runs. Likewise the carry in ripples past a
-// lane's first word only when that word is all ones, so the ripple through
-// the other words runs only when some lane needs it.
+// of lane l-1 (one lane shift of the wave, DPP wave_shr:1), and it cannot
+// ripple past the first word; otherwise the full generate/propagate
+// look-ahead runs, and the ripple through a lane's other words runs only
+// when some lane needs it.
 template <int W>
 __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], const uint32_t (&u)[W],
                                              uint32_t carry, int lane) {
   constexpr uint64_t KTOP = 1ull << 63;
-  uint32_t cin;
   if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) == 0) {
-    cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    // no first word is all ones: the carry in cannot ripple past it
+    const uint32_t cin =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    s[0] += cin;
   } else {
     const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
     uint32_t pall = ~0u;
@@ -445,16 +447,16 @@ __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W],
     const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
     const uint64_t X = G | P;
     const uint64_t C = (X + G) ^ X ^ G;
-    cin = (uint32_t)(C >> lane) & 1u;
-  }
-  uint32_t co;
-  s[0] = __builtin_addc(s[0], 0u, cin, &co);
-  if (W > 1 && __builtin_amdgcn_ballot_w64(co != 0u) != 0) {
+    const uint32_t cin = (uint32_t)(C >> lane) & 1u;
+    uint32_t co;
+    s[0] = __builtin_addc(s[0], 0u, cin, &co);
+    if (W > 1 && __builtin_amdgcn_ballot_w64(co != 0u) != 0) {
 #pragma unroll
-    for (int w = 1; w < W; ++w) {
-      uint32_t c2;
-      s[w] = __builtin_addc(s[w], 0u, co, &c2);
-      co = c2;
+      for (int w = 1; w < W; ++w) {
+        uint32_t c2;
+        s[w] = __builtin_addc(s[w], 0u, co, &c2);
+        co = c2;
+      }
     }
   }
 #pragma unroll
