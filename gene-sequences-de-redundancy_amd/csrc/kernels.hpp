@@ -85,7 +85,7 @@ constexpr int kMaxSpec = 64;
 constexpr int kFilterChunk = 16;
 // With one table per block (long words) a unit takes four candidates per
 // 16-lane group, whose record loads are issued together.
-__host__ __device__ constexpr int filter_subs(int group) { return group == 1 ? 4 : 1; }
+__host__ __device__ constexpr int filter_subs(int group) { return group <= 4 ? 4 : 1; }
 __host__ __device__ constexpr int filter_chunk(int group) { return kFilterChunk * filter_subs(group); }
 
 // Per-job round state on the device. The candidate list cand[cur][0..nc)

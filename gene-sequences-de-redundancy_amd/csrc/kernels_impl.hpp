@@ -104,7 +104,7 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
 // Waves per SIMD the register allocation must allow: the tables take 32 KiB
 // of LDS per block, so four blocks (four waves per SIMD) fit a CU; the
 // widest group is held to that too.
-__host__ __device__ constexpr int filter_waves_per_eu(int g) { return g >= 8 ? 4 : 1; }
+__host__ __device__ constexpr int filter_waves_per_eu(int g) { return g >= 4 ? 4 : 1; }
 
 template <int G>
 __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(FilterArgs a) {
