@@ -161,9 +161,11 @@ StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, in
                             const Tuning& T = Tuning());
 // Alignment workgroups per CU for a tile shape under knobs T; 0 = the
 // shape kernel's resident blocks per CU (registers and LDS decide it).
-// Measured: resident grids help config 2 (many short pairs, 910 -> 776 ms)
-// and cost config 4 2 % (its long-pair kernel, which ends each step, then
-// shares the CUs with more blocks of the other shapes), so the default is 4.
+// Measured with 8 per CU against 4: many short pairs gain (200,000 records,
+// t=0.90: 736 -> 638 ms per clustering), while a job whose long-pair kernel
+// ends each step loses (2,000,000 records, t=0.95: 473 -> 510 ms: that
+// kernel then shares the CUs with more blocks of the other shapes), so the
+// default is 4 and the tuner picks per workload.
 inline int align_blocks_for(const Tuning& T, int shape) {
   return T.align_blocks > 0 ? T.align_blocks : gsdr::align_blocks_per_cu(shape);
 }
