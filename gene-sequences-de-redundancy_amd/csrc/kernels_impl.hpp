@@ -382,6 +382,40 @@ __device__ inline uint32_t valid_bits(int gw, int n) {
   return 0u;
 }
 
+// Truth tables of v_bitop3_b32: bit i of the immediate is the result for the
+// input bits (a, b, c) = (bit 2, bit 1, bit 0 of i), i.e. a = 0xF0, b = 0xCC,
+// c = 0xAA.
+constexpr int kBitAndXnor = 0xF0 & ~(0xCC ^ 0xAA) & 0xFF;  // a & ~(b ^ c)
+constexpr int kBitOrXor = (0xF0 | (0xCC ^ 0xAA)) & 0xFF;    // a | (b ^ c)
+
+// S = A + B on every lane, with the lanes whose add carried out returned as a
+// lane mask. One v_add_co_u32, whose carry out is that mask (the compiler
+// turns a ballot of the carry into two more vector operations).
+__device__ __forceinline__ uint32_t add_carry_out(uint32_t a, uint32_t b, uint64_t& carries) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t s;
+  asm("v_add_co_u32_e64 %0, %1, %2, %3" : "=v"(s), "=s"(carries) : "v"(a), "v"(b));
+  return s;
+#else
+  const uint32_t s = a + b;
+  carries = __builtin_amdgcn_ballot_w64(s < a);
+  return s;
+#endif
+}
+
+// A + 1 on the lanes set in the wave-uniform mask cin, A elsewhere. One
+// v_addc_co_u32 with cin as its carry in.
+__device__ __forceinline__ uint32_t add_carry_in(uint32_t a, uint64_t cin) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t s;
+  uint64_t unused;
+  asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(s), "=s"(unused) : "v"(a), "s"(cin));
+  return s;
+#else
+  return a + (uint32_t)((cin >> (threadIdx.x & 63)) & 1u);
+#endif
+}
+
 // First half of a row for a wave-uniform query base B (2-bit code): match
 // mask, U = V & M and the per-lane multi-word sum S = V + U with its carry
 // out. The code is fixed at compile time, so M & V is one three-input bit
@@ -486,7 +520,10 @@ struct FullRows { static constexpr bool value = false; };
 
 template <int S, int W>
 __device__ inline void align_loop(const AlignArgs& a) {
-  constexpr uint64_t KTOP = top_lanes<S>();
+  static_assert(W == 1, "the row code below holds one text word per lane");
+  // carries stop at every segment's top lane (the wave's own top lane drops
+  // its carry out anyway)
+  constexpr uint64_t kCut = S == 64 ? ~0ull : ~top_lanes<S>();
   const int lane = threadIdx.x & 63;
   const int sl = lane & (S - 1);
   const int seg0 = lane & ~(S - 1);
@@ -622,12 +659,9 @@ __device__ inline void align_loop(const AlignArgs& a) {
     }
 
     // ---- 32 rows, unrolled in chunks of UR rows with constant bit offsets
-    constexpr int UR = W <= 4 ? 32 : 8;
-    // Segments live in this block, and whether any of them ends its query
-    // inside it. Without such a segment every live row of the block is a
-    // real row, so the carry shortcut below needs only the ballot of
-    // all-ones sums restricted to the live segments (a scalar mask).
-    const uint64_t live = __builtin_amdgcn_ballot_w64(active);
+    constexpr int UR = 8;
+    // Whether any segment ends its query inside this block: only such a
+    // block needs the query-end mask on its rows.
     const bool ragged = __builtin_amdgcn_ballot_w64(active && qnv != 0u) != 0;
     auto run_rows = [&](auto tag) {
       constexpr bool kRagged = decltype(tag)::value;
@@ -638,52 +672,24 @@ __device__ inline void align_loop(const AlignArgs& a) {
           // bit r as an all-zeros / all-ones mask (one signed bit-field extract)
           const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
           const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
-          // (without a query end in the block, live rows have qn = 0)
-          const uint32_t qn = kRagged ? (uint32_t)((int32_t)(qnc << (31 - r)) >> 31) : 0u;
-          uint32_t x[W], s[W];
-          uint32_t carry = 0u, pall = ~0u;
-#pragma unroll
-          for (int w = 0; w < W; ++w) {
-            x[w] = (tl[w] ^ ql) | (th[w] ^ qh) | qn;
-            const uint32_t u = v[w] & ~x[w];
-            uint32_t co;
-            s[w] = __builtin_addc(v[w], u, carry, &co);
-            carry = co;
-            pall &= s[w];
-          }
-          // A lane passes a carry on only when its whole sum is ones. When
-          // no lane of a live row has that, the carry into lane l is the
-          // carry out of lane l-1 (one lane shift; none into a segment's
-          // first lane). Rows past a query end and segments without a pair
-          // change nothing, so they do not count.
-          uint32_t cin;
-          bool common;
-          if constexpr (kRagged) {
-            common = __builtin_amdgcn_ballot_w64(active && qn == 0u && pall == ~0u) == 0;
-          } else {
-            common = (__builtin_amdgcn_ballot_w64(pall == ~0u) & live) == 0;
-          }
-          if (common) {
-            // lane 0 of the wave keeps the old value 0; a segment's first
-            // lane of a narrower tile is cleared
-            cin = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)carry, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-            if constexpr (S < 64) {
-              if (sl == 0) cin = 0u;
-            }
-          } else {
-            const uint64_t G = __builtin_amdgcn_ballot_w64(carry != 0u) & ~KTOP;
-            const uint64_t P = __builtin_amdgcn_ballot_w64(pall == ~0u) & ~KTOP;
-            const uint64_t X = G | P;
-            const uint64_t C = (X + G) ^ X ^ G;
-            cin = (uint32_t)(C >> lane) & 1u;
-          }
-#pragma unroll
-          for (int w = 0; w < W; ++w) {
-            uint32_t co;
-            const uint32_t t = __builtin_addc(s[w], 0u, cin, &co);
-            cin = co;
-            v[w] = t | (v[w] & x[w]);
-          }
+          // U = V & M, M = [text base == query base] = ~(tl ^ ql) & ~(th ^ qh):
+          // two three-input bit operations; rows past a query end match nothing
+          uint32_t u = __builtin_amdgcn_bitop3_b32(
+              __builtin_amdgcn_bitop3_b32(v[0], tl[0], ql, kBitAndXnor), th[0], qh, kBitAndXnor);
+          if constexpr (kRagged) u &= ~(uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
+          // S = V + U per lane (carry-out lanes G), then the carries between
+          // lanes by look-ahead on the scalar unit: a lane passes a carry on
+          // only where S is all ones (P), so the carries into the lanes are
+          // ((G|P) + G) ^ (G|P) ^ G, with every segment's top lane cut off.
+          // Rows past a query end and segments without a pair have U = 0 and
+          // generate nothing.
+          uint64_t g;
+          const uint32_t s = add_carry_out(v[0], u, g);
+          const uint64_t p = __builtin_amdgcn_ballot_w64(s == ~0u);
+          const uint64_t gm = g & kCut, xm = (g | p) & kCut;
+          const uint64_t cin = (xm + gm) ^ xm ^ gm;
+          // V' = (S + carry in) | (V ^ U)
+          v[0] = __builtin_amdgcn_bitop3_b32(add_carry_in(s, cin), v[0], u, kBitOrXor);
         }
       }
     };
