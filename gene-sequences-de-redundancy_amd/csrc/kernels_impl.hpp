@@ -439,23 +439,6 @@ __device__ __forceinline__ void row_sum_base(const uint32_t (&v)[W], const uint3
   asm volatile("" : "+v"(carry));
 }
 
-// The same first half with the base applied as two wave-uniform masks
-// (ql, qh all ones where the base bit is set): no branch, two more bit
-// operations per word. Cheaper than branching when a lane holds few words.
-template <int W>
-__device__ __forceinline__ void row_sum_masks(const uint32_t (&v)[W], const uint32_t (&tl)[W],
-                                              const uint32_t (&th)[W], uint32_t ql, uint32_t qh,
-                                              uint32_t (&s)[W], uint32_t (&u)[W], uint32_t& carry) {
-  carry = 0u;
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    u[w] = v[w] & ~(tl[w] ^ ql) & ~(th[w] ^ qh);
-    uint32_t co;
-    s[w] = __builtin_addc(v[w], u[w], carry, &co);
-    carry = co;
-  }
-}
-
 // Second half of a row for one 64-lane segment: carry look-ahead across the
 // lanes, then V' = (S + carry in) | (V ^ U). A lane passes a carry in on only
 // when all of its words of S are ones, which needs its first word to be all
@@ -495,6 +478,92 @@ __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W],
   }
 #pragma unroll
   for (int w = 0; w < W; ++w) v[w] = s[w] | (v[w] ^ u[w]);
+}
+
+// S = V + U over a lane's W words (word 0 lowest) as one v_add_co / v_addc
+// chain, returning the carry out of the top word as a lane mask. A carry
+// written by one vector operation may be read as the next one's carry in only
+// two wait states later, hence the s_nop between the links.
+template <int W>
+__device__ __forceinline__ uint64_t add_words(const uint32_t (&v)[W], const uint32_t (&u)[W], uint32_t (&s)[W]) {
+  static_assert(W >= 2 && W <= 4, "chains of 2 to 4 words");
+  uint64_t g;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (W == 2) {
+    asm("v_add_co_u32_e64 %0, %2, %3, %5\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %2, %4, %6, %2"
+        : "=&v"(s[0]), "=&v"(s[1]), "=&s"(g)
+        : "v"(v[0]), "v"(v[1]), "v"(u[0]), "v"(u[1]));
+  } else if constexpr (W == 3) {
+    asm("v_add_co_u32_e64 %0, %3, %4, %7\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %3, %5, %8, %3\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %2, %3, %6, %9, %3"
+        : "=&v"(s[0]), "=&v"(s[1]), "=&v"(s[2]), "=&s"(g)
+        : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(u[0]), "v"(u[1]), "v"(u[2]));
+  } else {
+    asm("v_add_co_u32_e64 %0, %4, %5, %9\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %4, %6, %10, %4\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %2, %4, %7, %11, %4\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %3, %4, %8, %12, %4"
+        : "=&v"(s[0]), "=&v"(s[1]), "=&v"(s[2]), "=&v"(s[3]), "=&s"(g)
+        : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(u[0]), "v"(u[1]), "v"(u[2]), "v"(u[3]));
+  }
+#else
+  uint32_t c = 0u;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint32_t co;
+    s[w] = __builtin_addc(v[w], u[w], c, &co);
+    c = co;
+  }
+  g = __builtin_amdgcn_ballot_w64(c != 0u);
+#endif
+  return g;
+}
+
+// A whole row of a 64-lane tile with 2 to 4 words per lane and the query base
+// as two wave-uniform masks (ql, qh all ones where the base bit is set).
+// U = V & M from two v_bitop3 per word; S = V + U by add_words, whose carry
+// out is already a lane mask G; then the carries between lanes on the scalar
+// unit. A lane passes a carry on only when all of its words of S are ones,
+// which needs its first word to be all ones: when no lane has that, the carry
+// into lane l is the carry out of lane l-1 (G << 1) and stops in the first
+// word; otherwise the full look-ahead ((G|P) + G) ^ (G|P) ^ G runs, with the
+// ripple through the lane's words. V' = (S + carry in) | (V ^ U).
+template <int W>
+__device__ __forceinline__ void row_masks64(uint32_t (&v)[W], const uint32_t (&tl)[W], const uint32_t (&th)[W],
+                                            uint32_t ql, uint32_t qh, int lane) {
+  uint32_t u[W], s[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    u[w] = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(v[w], tl[w], ql, kBitAndXnor), th[w], qh,
+                                       kBitAndXnor);
+  const uint64_t g = add_words<W>(v, u, s);
+  if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) == 0) {
+    s[0] = add_carry_in(s[0], g << 1);
+  } else {
+    uint32_t pall = ~0u;
+#pragma unroll
+    for (int w = 0; w < W; ++w) pall &= s[w];
+    const uint64_t p = __builtin_amdgcn_ballot_w64(pall == ~0u);
+    const uint64_t x = g | p;
+    const uint64_t cm = (x + g) ^ x ^ g;
+    uint32_t co = (uint32_t)(cm >> lane) & 1u;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint32_t c2;
+      s[w] = __builtin_addc(s[w], 0u, co, &c2);
+      co = c2;
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) v[w] = __builtin_amdgcn_bitop3_b32(s[w], v[w], u[w], kBitOrXor);
 }
 
 // Banded bit-parallel LCS decision, LCS(text, query) > bc_query.
@@ -880,8 +949,8 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     const int nrows = min(32, mq - jb * 32);
 #pragma unroll 1
     for (int r = 0; r < nrows; ++r) {
-      uint32_t s[W], u[W], carry;
       if constexpr (W >= 6) {
+        uint32_t s[W], u[W], carry;
         // With 6 or more words per lane each row branches (scalar) to the
         // code specialised for its base; the empty asm keeps the four match
         // masks from being hoisted out of the loop (4*W more registers).
@@ -892,15 +961,15 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
         else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
         else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
         else row_sum_base<3, W>(v, tl, th, s, u, carry);
+        row_finish64<W>(v, s, u, carry, lane);
       } else {
         // with fewer words the base enters as uniform masks: measured faster
         // there than the branch, whose scalar issue outweighs the two bit
         // operations per word it saves
-        const uint32_t ql = ((qls >> r) & 1u) ? ~0u : 0u;
-        const uint32_t qh = ((qhs >> r) & 1u) ? ~0u : 0u;
-        row_sum_masks<W>(v, tl, th, ql, qh, s, u, carry);
+        const uint32_t ql = (uint32_t)((int32_t)(qls << (31 - r)) >> 31);
+        const uint32_t qh = (uint32_t)((int32_t)(qhs << (31 - r)) >> 31);
+        row_masks64<W>(v, tl, th, ql, qh, lane);
       }
-      row_finish64<W>(v, s, u, carry, lane);
     }
     if (lane == 0) cells += 64ull * W * 32ull * 32ull;
 
