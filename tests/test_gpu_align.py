@@ -123,6 +123,34 @@ def test_single_base_runs(engine, shape):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("shape,cap", [(0, 255), (1, 511), (2, 1023), (3, 2047), (6, 8000)])
+def test_single_base_runs_narrow(engine, shape, cap):
+    """The same single-base-run texts sized so that the one-word tiles (8, 16,
+    32 and 64 lanes of one word) and the 64x4 tile hold them: all-ones sums
+    and carry chains across lanes and segment boundaries in every row, with
+    queries ending inside 32-row blocks."""
+    rng = np.random.default_rng(70 + shape)
+    seqs, pairs = [], []
+    for i in range(16):
+        runs, n = [], int(rng.integers(cap // 3, cap))
+        while sum(len(r) for r in runs) < n:
+            runs.append(bytes([int(rng.choice(ACGT))]) * int(rng.integers(33, 200)))
+        text = b"".join(runs)[:n]
+        if i % 4 == 0:
+            q = rand_seq(rng, int(rng.integers(20, max(21, n // 2))))
+        elif i % 4 == 1:
+            q = mutate(rng, text, 0.05)
+        elif i % 4 == 2:
+            q = text[: n // 3] + rand_seq(rng, 40) + text[n // 2:]
+        else:
+            q = bytes(reversed(text))
+        seqs += [text, q]
+        pairs.append((2 * i, 2 * i + 1))
+    ep, bcs, expect = boundary_cases(seqs, pairs)
+    got, _ = engine.align_pairs(seqs, ep, bcs, shape=shape)
+    assert np.array_equal(got, expect)
+
+
 def test_near_threshold_long(engine):
     """Long similar pairs at t=0.95/0.90 cutoffs and the 32,767-base maximum."""
     rng = np.random.default_rng(7)
