@@ -1127,10 +1127,11 @@ __device__ inline void align_loop64(const AlignArgs& a) {
 }
 
 // Waves per SIMD the register allocation must allow (HIP's second launch
-// bound): 4 up to 10 words per lane, then 3 and 2 (measured: the widest
-// tiles cannot fit more without spilling).
+// bound): 8 for the narrow one-word tiles (62 registers, no spills; lets a
+// grid of 8 blocks per CU be resident), 4 up to 10 words per lane, then 3
+// and 2 (measured: the widest tiles cannot fit more without spilling).
 __host__ __device__ constexpr int align_waves_per_eu(int s, int w) {
-  return s < 64 || w <= 10 ? 4 : w <= 12 ? 3 : 2;
+  return s < 64 ? 8 : w <= 10 ? 4 : w <= 12 ? 3 : 2;
 }
 
 template <int S, int W>
