@@ -748,15 +748,18 @@ __device__ inline void align_loop(const AlignArgs& a) {
           if constexpr (kRagged) u &= ~(uint32_t)((int32_t)(qnc << (31 - r)) >> 31);
           // S = V + U per lane (carry-out lanes G), then the carries between
           // lanes by look-ahead on the scalar unit: a lane passes a carry on
-          // only where S is all ones (P), so the carries into the lanes are
-          // ((G|P) + G) ^ (G|P) ^ G, with every segment's top lane cut off.
-          // Rows past a query end and segments without a pair have U = 0 and
-          // generate nothing.
+          // only where S is all ones (P), which never holds where it
+          // generates one (V + U < 2^33 - 1). With X = G | P, bit l of X + G
+          // is then P_l ^ (carry into lane l), so the carries into the lanes
+          // are (X + G) ^ P, with every segment's top lane cut off in G and P
+          // (four scalar operations; the 64-bit AND of the cut is free for a
+          // 64-lane segment). Rows past a query end and segments without a
+          // pair have U = 0 and generate nothing.
           uint64_t g;
           const uint32_t s = add_carry_out(v[0], u, g);
           const uint64_t p = __builtin_amdgcn_ballot_w64(s == ~0u);
-          const uint64_t gm = g & kCut, xm = (g | p) & kCut;
-          const uint64_t cin = (xm + gm) ^ xm ^ gm;
+          const uint64_t gm = g & kCut, pm = p & kCut;
+          const uint64_t cin = ((gm | pm) + gm) ^ pm;
           // V' = (S + carry in) | (V ^ U)
           v[0] = __builtin_amdgcn_bitop3_b32(add_carry_in(s, cin), v[0], u, kBitOrXor);
         }
