@@ -950,9 +950,9 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     const uint32_t qls = __builtin_amdgcn_readfirstlane(qw.x);
     const uint32_t qhs = __builtin_amdgcn_readfirstlane(qw.y);
     const int nrows = min(32, mq - jb * 32);
+    if constexpr (W >= 6) {
 #pragma unroll 1
-    for (int r = 0; r < nrows; ++r) {
-      if constexpr (W >= 6) {
+      for (int r = 0; r < nrows; ++r) {
         uint32_t s[W], u[W], carry;
         // With 6 or more words per lane each row branches (scalar) to the
         // code specialised for its base; the empty asm keeps the four match
@@ -965,10 +965,27 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
         else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
         else row_sum_base<3, W>(v, tl, th, s, u, carry);
         row_finish64<W>(v, s, u, carry, lane);
-      } else {
-        // with fewer words the base enters as uniform masks: measured faster
-        // there than the branch, whose scalar issue outweighs the two bit
-        // operations per word it saves
+      }
+    } else if (nrows == 32) {
+      // With fewer words the base enters as uniform masks: measured faster
+      // there than the branch, whose scalar issue outweighs the two bit
+      // operations per word it saves. A full block runs in chunks of 8 rows
+      // with constant bit offsets: each mask is one scalar bit-field
+      // extract and the loop counts chunks, not rows (the scalar unit also
+      // issues the carries' look-ahead, at the vector unit's rate).
+#pragma unroll 1
+      for (int r0 = 0; r0 < 32; r0 += 8) {
+        const uint32_t qlc = qls >> r0, qhc = qhs >> r0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const uint32_t ql = (uint32_t)((int32_t)(qlc << (31 - r)) >> 31);
+          const uint32_t qh = (uint32_t)((int32_t)(qhc << (31 - r)) >> 31);
+          row_masks64<W>(v, tl, th, ql, qh, lane);
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int r = 0; r < nrows; ++r) {
         const uint32_t ql = (uint32_t)((int32_t)(qls << (31 - r)) >> 31);
         const uint32_t qh = (uint32_t)((int32_t)(qhs << (31 - r)) >> 31);
         row_masks64<W>(v, tl, th, ql, qh, lane);
