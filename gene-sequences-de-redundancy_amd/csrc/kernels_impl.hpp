@@ -951,20 +951,31 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     const uint32_t qhs = __builtin_amdgcn_readfirstlane(qw.y);
     const int nrows = min(32, mq - jb * 32);
     if constexpr (W >= 6) {
-#pragma unroll 1
-      for (int r = 0; r < nrows; ++r) {
+      // With 6 or more words per lane each row branches (scalar) to the
+      // code specialised for its base; the empty asm keeps the four match
+      // masks from being hoisted out of the loop (4*W more registers).
+      auto row = [&](uint32_t base) {
         uint32_t s[W], u[W], carry;
-        // With 6 or more words per lane each row branches (scalar) to the
-        // code specialised for its base; the empty asm keeps the four match
-        // masks from being hoisted out of the loop (4*W more registers).
 #pragma unroll
         for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
-        const uint32_t base = ((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1);
         if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
         else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
         else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
         else row_sum_base<3, W>(v, tl, th, s, u, carry);
         row_finish64<W>(v, s, u, carry, lane);
+      };
+      if (W <= 8 && nrows == 32) {
+        // a full block in chunks of 8 rows with constant bit offsets (the
+        // wider tiles would spill more registers so, and keep the row loop)
+#pragma unroll 1
+        for (int r0 = 0; r0 < 32; r0 += 8) {
+          const uint32_t qlc = qls >> r0, qhc = qhs >> r0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) row(((qlc >> r) & 1u) | (((qhc >> r) & 1u) << 1));
+        }
+      } else {
+#pragma unroll 1
+        for (int r = 0; r < nrows; ++r) row(((qls >> r) & 1u) | (((qhs >> r) & 1u) << 1));
       }
     } else if (nrows == 32) {
       // With fewer words the base enters as uniform masks: measured faster
