@@ -1,0 +1,42 @@
+"""The alignment rows keep their carries as lane masks in scalar registers.
+
+The row code writes `v_add_co_u32` / `v_addc_co_u32` in inline asm with an "s"
+constraint for the carry out and the carry in. The compiler treats an asm
+output as divergent and, where such a value meets another at a join, has been
+seen to move it into vector registers and still print that operand for the "s"
+constraint; the assembler takes it and the carry in would read the wrong
+register. This compiles the kernels to gfx950 assembly and checks that every
+such instruction names scalar registers for both carries.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "gene-sequences-de-redundancy_amd")
+HIPCC = "/opt/rocm/bin/hipcc"
+
+SCALAR = re.compile(r"^(s\[\d+:\d+\]|vcc)$")
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_asm_carries_are_scalar(tmp_path):
+    out = tmp_path / "kernels.s"
+    cmd = [HIPCC, "-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "--cuda-device-only", "-S", "-o", str(out), os.path.join(PKG, "csrc", "kernels.hip")]
+    r = subprocess.run(cmd, cwd=PKG, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    n = 0
+    for line in out.read_text().splitlines():
+        t = line.split()
+        if not t or t[0] not in ("v_add_co_u32_e64", "v_addc_co_u32_e64"):
+            continue
+        ops = [o.rstrip(",") for o in t[1:]]
+        n += 1
+        assert SCALAR.match(ops[1]), f"carry out not in scalar registers: {line.strip()}"
+        if t[0] == "v_addc_co_u32_e64":
+            assert SCALAR.match(ops[-1]), f"carry in not in scalar registers: {line.strip()}"
+    assert n > 0, "no add-with-carry instructions found"
