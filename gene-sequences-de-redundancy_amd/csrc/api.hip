@@ -60,6 +60,75 @@ const char* gsdr_strerror(int32_t status) {
 }
 const char* gsdr_last_error(void) { return g_last_error.c_str(); }
 
+// ----------------------------------------------------------------- options
+// checkOption (func.cpp:6-57) without the exit: the messages the reference
+// prints before exiting with status 0 are returned as the error text.
+int gsdr_parse_options(int argc, char** argv, gsdr_options* out) {
+  static const char kUsage[] = "use like this: cluster i inputFile t threshold";  // func.cpp:3
+  if (!out || argc < 1 || !argv) return fail(GSDR_E_INVALID, "null argument");
+  gsdr_options o;
+  std::memset(&o, 0, sizeof(o));
+  o.input = "testData.fasta";  // func.cpp:8-11
+  o.output = "result.fasta";
+  o.threshold = 0.95f;
+  o.word_length = 0;
+  o.mode = GSDR_MODE_BATCHED;
+  for (int i = 1; i < argc; i += 2) {
+    if (i + 1 >= argc || !argv[i] || !argv[i + 1]) return fail(GSDR_E_INVALID, kUsage);  // func.cpp:7
+    const std::string key = argv[i];
+    const char* val = argv[i + 1];
+    if (key.rfind("--", 0) == 0) {
+      const std::string v = val;
+      if (key == "--mode" && (v == "plain" || v == "batched")) o.mode = v == "plain" ? GSDR_MODE_PLAIN : GSDR_MODE_BATCHED;
+      else if (key == "--spec") o.spec_reps = std::atoi(val);
+      else if (key == "--membership") o.membership = val;
+      else if (key == "--report") o.report = val;
+      else if (key == "--device") o.device = std::atoi(val);
+      else if (key == "--drop" && (v == "0" || v == "1")) o.drop_filters = v == "1";
+      else return fail(GSDR_E_INVALID, kUsage);
+      continue;
+    }
+    // the reference switches on the first letter (func.cpp:13-29); a
+    // leading '-' is accepted as well
+    const char* k = key.c_str();
+    if (k[0] == '-' && k[1] != 0) ++k;
+    try {
+      switch (k[0]) {
+        case 'i': o.input = val; break;
+        case 'o': o.output = val; break;
+        case 't': o.threshold = std::stof(val); break;
+        case 'w': o.word_length = std::stoi(val); break;
+        default: return fail(GSDR_E_INVALID, kUsage);
+      }
+    } catch (...) {
+      return fail(GSDR_E_INVALID, kUsage);
+    }
+  }
+  if (o.threshold < 0.8 || o.threshold >= 1)  // func.cpp:31-35 (float against double literals)
+    return fail(GSDR_E_INVALID, std::string("similarity out of range\n0.8<=similarity<1\n") + kUsage);
+  if (o.word_length == 0) {
+    o.word_length = auto_word_length(o.threshold);  // func.cpp:36-45
+  } else if (o.word_length < 4 || o.word_length > 8) {  // func.cpp:46-51
+    return fail(GSDR_E_INVALID, std::string("word length out of range\n4<=word length<=8\n") + kUsage);
+  }
+  *out = o;
+  return GSDR_OK;
+}
+
+// ----------------------------------------------------------------- cutoffs
+int gsdr_cutoffs(float threshold, int32_t word_length, const int32_t* raw_len, int64_t n, int32_t* bc_out,
+                 int32_t* wc_out) {
+  if (n < 0 || (n > 0 && (!raw_len || !bc_out || !wc_out))) return fail(GSDR_E_INVALID, "null argument");
+  const int k = word_length == 0 ? auto_word_length(threshold) : word_length;
+  if (k < 4 || k > 8) return fail(GSDR_E_INVALID, "word length must be 4..8");
+  for (int64_t i = 0; i < n; ++i) {
+    if (raw_len[i] < 0) return fail(GSDR_E_INVALID, "negative length");
+    bc_out[i] = base_cutoff(raw_len[i], threshold);
+    wc_out[i] = word_cutoff(raw_len[i], threshold, k);
+  }
+  return GSDR_OK;
+}
+
 // ----------------------------------------------------------------- records
 static int read_records(const char* path, bool length_order, gsdr_records** out) {
   if (!path || !out) return fail(GSDR_E_INVALID, "null argument");
@@ -180,7 +249,13 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
   HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   c->cu_count = prop.multiProcessorCount;
   c->hbm_bytes = (int64_t)prop.totalGlobalMem;
-  snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+  // some driver stacks report an empty marketing name: fall back to the
+  // architecture and compute-unit count
+  if (prop.name[0] != 0)
+    snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+  else
+    snprintf(c->name, sizeof(c->name), "%s, %d CUs, %.0f GiB", prop.gcnArchName, prop.multiProcessorCount,
+             (double)prop.totalGlobalMem / (double)(1ll << 30));
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return fail(GSDR_E_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");

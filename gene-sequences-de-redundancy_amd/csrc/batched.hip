@@ -42,8 +42,8 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   return L.ra.njobs > 1 ? launch_clamp_spec(L.ra.jobs, L.ra.njobs, L.ra.pair_budget, q) : hipSuccess;
 }
 
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, int32_t* jobs_left, const Tuning& T) {
-  Workspace& w = c->ws;
+StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
+                            int32_t* jobs_left, const Tuning& T) {
   StepLaunch L;
   L.kmask = kmask;
   L.smax = smax;
@@ -90,7 +90,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, in
 }
 
 int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
-                    int32_t* const* out, gsdr_stats* stats) {
+                    int32_t* const* out, gsdr_stats* stats, const Workspace* fixed) {
   if (njobs == 0) {
     if (stats) std::memset(stats, 0, sizeof(*stats));
     return GSDR_OK;
@@ -101,7 +101,10 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   int smax = 0, timing = 0, spec = 0, tune = 0;
   unsigned kmask = 0;
   int64_t total_n = 0, total_bases = 0, max_n = 0;
-  std::vector<JobDev> jobs((size_t)njobs);
+  // host bookkeeping: on the stack for one job (the no-allocation entry)
+  JobDev job1[1];
+  std::vector<JobDev> jobs_many((size_t)(njobs > 1 ? njobs : 0));
+  JobDev* jobs = njobs > 1 ? jobs_many.data() : job1;
   for (int j = 0; j < njobs; ++j) {
     gsdr_store& s = *stores[j];
     if (s.ctx != c) return fail(GSDR_E_INVALID, "store belongs to another context");
@@ -127,23 +130,30 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   // representatives over every job's candidates
   // (the resolve step bounds each job's k_spec so that k_spec * nc of all
   // jobs fits, so the capacity also caps the speculation width)
-  const int64_t item_cap =
+  int64_t item_cap =
       std::min<int64_t>(std::max<int64_t>((int64_t)T.list_factor * total_n, max_n), 64ll << 20);
-  if ((st = workspace_reserve(c, item_cap, njobs))) return st;
+  if (fixed) {
+    if (timing) return fail(GSDR_E_INVALID, "timing needs HIP events, which this entry does not create");
+    if (njobs > fixed->jobs_cap) return fail(GSDR_E_LIMIT, "too many jobs for the scratch");
+    item_cap = fixed->item_cap;
+  } else if ((st = workspace_reserve(c, item_cap, njobs))) {
+    return st;
+  }
+  const Workspace w = fixed ? *fixed : c->ws;
   // the first step queues every job's whole candidate list once
-  if (total_n > c->ws.item_cap) return fail(GSDR_E_LIMIT, "too many sequences in one call");
-  Workspace& w = c->ws;
+  if (total_n > w.item_cap)
+    return fail(GSDR_E_LIMIT, fixed ? "too many sequences for the scratch" : "too many sequences in one call");
   hipStream_t q = c->stream;
   const auto t0 = std::chrono::steady_clock::now();
   int32_t left = 0;
-  for (auto& jd : jobs) left += jd.n > 0;
-  HIP_TRY(hipMemcpyAsync(w.jobs, jobs.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice, q), "job table");
+  for (int j = 0; j < njobs; ++j) left += jobs[j].n > 0;
+  HIP_TRY(hipMemcpyAsync(w.jobs, jobs, sizeof(JobDev) * njobs, hipMemcpyHostToDevice, q), "job table");
   HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
   HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
-  HIP_TRY(launch_init_jobs(w.jobs, jobs.data(), njobs, k_max, w.jobs_left, q), "init");
+  HIP_TRY(launch_init_jobs(w.jobs, jobs, njobs, k_max, w.jobs_left, q), "init");
 
-  StepLaunch L = make_step_launch(c, njobs, smax, kmask, w.jobs_left, T);
+  StepLaunch L = make_step_launch(c, w, njobs, smax, kmask, w.jobs_left, T);
   // the speculation bound follows this call's capacity, not a larger
   // workspace left by an earlier call
   L.ra.pair_budget = std::min<int64_t>(item_cap, w.item_cap);
@@ -174,6 +184,20 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       if (steps_launched > step_limit) return fail(GSDR_E_DEVICE, "steps did not converge");
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+  } else if (fixed) {
+    // no graph (instantiating one allocates): the same steps launched
+    // directly, one host synchronisation per B steps
+    const int B = 16;
+    int32_t* h_left = reinterpret_cast<int32_t*>(c->h_state);
+    while (true) {
+      for (int b = 0; b < B; ++b) HIP_TRY(launch_step(c, L, q, nullptr), "step launch");
+      HIP_TRY(hipMemcpyAsync(h_left, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
+      HIP_TRY(hipStreamSynchronize(q), "step sync");
+      steps_launched += B;
+      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0));
+      if (*h_left == 0) break;
+      if (steps_launched > step_limit + B) return fail(GSDR_E_DEVICE, "steps did not converge");
+    }
   } else {
     // graph form: B steps per replay, one host synchronisation per replay
     const int B = 16;
@@ -212,7 +236,9 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
             "cluster copy");
   unsigned long long hst[ST_NUM];
   HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
-  std::vector<JobState> hs((size_t)njobs);
+  JobState hs1[1];
+  std::vector<JobState> hs_many((size_t)(njobs > 1 ? njobs : 0));
+  JobState* hs = njobs > 1 ? hs_many.data() : hs1;
   for (int j = 0; j < njobs; ++j)
     HIP_TRY(hipMemcpyAsync(&hs[j], jobs[j].state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");

@@ -24,66 +24,14 @@
 
 namespace {
 
-[[noreturn]] void usage() {
-  std::cout << "use like this: cluster i inputFile t threshold" << std::endl;
-  std::exit(0);  // the reference exits with status 0 on usage errors (func.cpp:2-5)
-}
-
-struct Options {
-  std::string in = "testData.fasta", out = "result.fasta", membership, report;
-  float t = 0.95f;
-  int k = 0;
-  int mode = GSDR_MODE_BATCHED;
-  int spec = 0;
-  int device = 0;
-  int drop = 0;
-};
-
-Options parse(int argc, char** argv) {
-  Options o;
-  int i = 1;
-  while (i < argc) {
-    std::string key = argv[i];
-    if (key.rfind("--", 0) == 0) {
-      if (i + 1 >= argc) usage();
-      std::string val = argv[i + 1];
-      if (key == "--mode") o.mode = (val == "plain") ? GSDR_MODE_PLAIN : (val == "batched" ? GSDR_MODE_BATCHED : -1);
-      else if (key == "--spec") o.spec = std::atoi(val.c_str());
-      else if (key == "--membership") o.membership = val;
-      else if (key == "--report") o.report = val;
-      else if (key == "--device") o.device = std::atoi(val.c_str());
-      else if (key == "--drop") o.drop = (val == "1") ? 1 : (val == "0" ? 0 : -1);
-      else usage();
-      if (o.mode < 0 || o.drop < 0) usage();
-      i += 2;
-      continue;
-    }
-    if (i + 1 >= argc) usage();
-    const char* k = argv[i];
-    if (k[0] == '-' && k[1] != 0) ++k;
-    try {
-      switch (k[0]) {
-        case 'i': o.in = argv[i + 1]; break;
-        case 'o': o.out = argv[i + 1]; break;
-        case 't': o.t = std::stof(argv[i + 1]); break;
-        case 'w': o.k = std::stoi(argv[i + 1]); break;
-        default: usage();
-      }
-    } catch (...) {
-      usage();
-    }
-    i += 2;
-  }
-  // func.cpp:31-52
-  if (o.t < 0.8 || o.t >= 1) {
-    std::cout << "similarity out of range" << std::endl << "0.8<=similarity<1" << std::endl;
-    usage();
-  }
-  if (o.k == 0) {
-    o.k = gsdr::auto_word_length(o.t);
-  } else if (o.k < 4 || o.k > 8) {
-    std::cout << "word length out of range" << std::endl << "4<=word length<=8" << std::endl;
-    usage();
+// Options through the C ABI's parser (checkOption, func.cpp:6-57); on a
+// usage or range error the reference prints its message and exits with
+// status 0 (func.cpp:2-5).
+gsdr_options parse(int argc, char** argv) {
+  gsdr_options o;
+  if (gsdr_parse_options(argc, argv, &o) != GSDR_OK) {
+    std::cout << gsdr_last_error() << std::endl;
+    std::exit(0);
   }
   return o;
 }
@@ -97,11 +45,12 @@ int die(const char* what, int st) {
 
 int main(int argc, char** argv) {
   const auto t0 = std::chrono::steady_clock::now();
-  Options o = parse(argc, argv);
-  std::cout << "input file:\t" << o.in << std::endl;
-  std::cout << "output file:\t" << o.out << std::endl;
-  std::cout << "similarity:\t" << o.t << std::endl;
-  std::cout << "word length:\t" << o.k << std::endl;
+  const gsdr_options o = parse(argc, argv);
+  // func.cpp:53-56
+  std::cout << "input file:\t" << o.input << std::endl;
+  std::cout << "output file:\t" << o.output << std::endl;
+  std::cout << "similarity:\t" << o.threshold << std::endl;
+  std::cout << "word length:\t" << o.word_length << std::endl;
 
   // Device start-up (runtime init, streams) overlaps reading and sorting
   // the input on the calling thread.
@@ -117,7 +66,7 @@ int main(int argc, char** argv) {
   gsdr_records* recs = nullptr;
   // one pass puts the records straight into the length order
   // (func.cpp:70-107); sort then only reports the permutation
-  int st = gsdr_records_read_fasta_sorted(o.in.c_str(), &recs);
+  int st = gsdr_records_read_fasta_sorted(o.input, &recs);
   const auto t_read = std::chrono::steady_clock::now();
   const int64_t n = st ? 0 : gsdr_records_count(recs);
   std::vector<int64_t> perm((size_t)n);
@@ -132,22 +81,23 @@ int main(int argc, char** argv) {
   int64_t first_len = 0, last_len = 0;
   gsdr_records_get(recs, 0, nullptr, nullptr, nullptr, &first_len);
   gsdr_records_get(recs, n - 1, nullptr, nullptr, nullptr, &last_len);
+  // func.cpp:108-111
   std::cout << "read file complete" << std::endl;
-  std::cout << "longest/shortest:\t" << first_len << "/" << last_len << std::endl;
-  std::cout << "reads count:\t" << n << std::endl;
+  std::cout << "longest/shortest\uff1a\t" << first_len << "/" << last_len << std::endl;
+  std::cout << "reads count\uff1a\t" << n << std::endl;
 
   gsdr_store* store = nullptr;
   const auto t_store0 = std::chrono::steady_clock::now();
-  if ((st = gsdr_load(ctx, recs, o.k, &store))) return die("load", st);
+  if ((st = gsdr_load(ctx, recs, o.word_length, &store))) return die("load", st);
   const auto t1 = std::chrono::steady_clock::now();
 
   gsdr_params p;
   std::memset(&p, 0, sizeof(p));
-  p.threshold = o.t;
-  p.word_length = o.k;
+  p.threshold = o.threshold;
+  p.word_length = o.word_length;
   p.mode = o.mode;
-  p.spec_reps = o.spec;
-  p.drop_filters = o.drop;
+  p.spec_reps = o.spec_reps;
+  p.drop_filters = o.drop_filters;
   std::vector<int32_t> cl((size_t)n);
   gsdr_stats stats;
   std::memset(&stats, 0, sizeof(stats));
@@ -155,18 +105,19 @@ int main(int argc, char** argv) {
 
   const auto t_clu = std::chrono::steady_clock::now();
   int64_t reps = 0;
-  if ((st = gsdr_records_write_result(recs, cl.data(), o.out.c_str(), &reps))) return die("write", st);
-  if (!o.membership.empty() &&
-      (st = gsdr_records_write_membership(recs, perm.data(), cl.data(), o.membership.c_str())))
+  if ((st = gsdr_records_write_result(recs, cl.data(), o.output, &reps))) return die("write", st);
+  if (o.membership &&
+      (st = gsdr_records_write_membership(recs, perm.data(), cl.data(), o.membership)))
     return die("membership", st);
   const auto t2 = std::chrono::steady_clock::now();
   auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
   const double wall = secs(t0, t2);
   const double dev = stats.ms_total / 1e3;
-  std::cout << "cluster:\t" << reps << std::endl;
-  std::cout << "time consuming:\t" << (long long)(wall * 1000.0) << "ms" << std::endl;
-  if (!o.report.empty()) {
-    FILE* f = std::fopen(o.report.c_str(), "w");
+  // func.cpp:798 and main.cpp:26 (wall time here, not CPU time)
+  std::cout << "cluster\uff1a" << reps << std::endl;
+  std::cout << "time consuming\uff1a" << (long long)(wall * 1000.0) << "ms" << std::endl;
+  if (o.report) {
+    FILE* f = std::fopen(o.report, "w");
     if (f) {
       std::fprintf(f,
                    "{\"n_seqs\": %lld, \"n_bases\": %lld, \"reps\": %lld, \"rounds\": %lld, "

@@ -46,8 +46,10 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
   if ((st = dev_alloc(A, F, &s.bc, (size_t)n)) || (st = dev_alloc(A, F, &s.wc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
-      (st = dev_alloc(A, F, &s.state, (size_t)1)))
+      (st = dev_alloc(A, F, &s.state, (size_t)1)) || (st = dev_alloc(A, F, &s.raw_len_d, (size_t)n)))
     return st;
+  HIP_TRY(hipMemcpy(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice),
+          "upload lengths");
   return GSDR_OK;
 }
 
@@ -94,18 +96,12 @@ void workspace_release(gsdr_ctx* c) {
   w = Workspace();
 }
 
-// Cutoffs for threshold t (func.cpp:339-374), computed on the host with the
-// reference's float32 expressions and uploaded.
+// Cutoffs for threshold t (func.cpp:339-374), computed on the device from
+// the stored raw lengths with the reference's float32 expressions.
 int store_set_threshold(gsdr_store& s, float t) {
   if (s.t_loaded == t) return GSDR_OK;
-  std::vector<int32_t> bc((size_t)s.n), wc((size_t)s.n);
-  for (int32_t i = 0; i < s.n; ++i) {
-    bc[i] = base_cutoff(s.raw_len[i], t);
-    wc[i] = word_cutoff(s.raw_len[i], t, s.k);
-  }
   hipStream_t q = s.ctx->upload;
-  HIP_TRY(hipMemcpyAsync(s.bc, bc.data(), bc.size() * 4, hipMemcpyHostToDevice, q), "upload bc");
-  HIP_TRY(hipMemcpyAsync(s.wc, wc.data(), wc.size() * 4, hipMemcpyHostToDevice, q), "upload wc");
+  HIP_TRY(launch_cutoffs(s.raw_len_d, s.n, t, s.k, s.bc, s.wc, q), "cutoff launch");
   HIP_TRY(hipStreamSynchronize(q), "cutoff sync");
   s.t_loaded = t;
   return GSDR_OK;
@@ -182,16 +178,23 @@ int max_shape_for(int n) {
 // representative can need, and the resolve/compaction step, then reads back
 // the job state (one synchronisation per round, like the reference's
 // per-round waits at func.cpp:695-781 but without its host compaction).
-int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
+int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats,
+                  const Workspace* fixed) {
   int k = 0;
   int st = check_params(s, p, &k);
   if (st) return st;
   HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
   if ((st = store_set_threshold(s, p.threshold))) return st;
-  if ((st = workspace_reserve(c, std::max<int64_t>(s.n, 1), 1))) return st;
+  if (fixed) {
+    // a round queues at most one item per candidate in each shape list
+    if (p.timing) return fail(GSDR_E_INVALID, "timing needs HIP events, which this entry does not create");
+    if (std::max<int64_t>(s.n, 1) > fixed->item_cap) return fail(GSDR_E_LIMIT, "too many sequences for the scratch");
+  } else if ((st = workspace_reserve(c, std::max<int64_t>(s.n, 1), 1))) {
+    return st;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t q = c->stream;
-  Workspace& w = c->ws;
+  const Workspace w = fixed ? *fixed : c->ws;
   const int n = s.n;
   const JobDev jd = s.job(p);
   HIP_TRY(hipMemcpyAsync(w.jobs, &jd, sizeof(JobDev), hipMemcpyHostToDevice, q), "job table");

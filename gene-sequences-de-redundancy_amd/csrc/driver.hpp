@@ -59,6 +59,9 @@ struct gsdr_ctx {
   hipStream_t side[gsdr::kNumShapes] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[gsdr::kNumShapes] = {};
+  // device address of this device's no-allocation scratch array
+  // (gsdr_cluster_noalloc), looked up when the context opens
+  void* scratch = nullptr;
 };
 
 struct gsdr_store {
@@ -77,6 +80,7 @@ struct gsdr_store {
   int64_t* kpoff = nullptr;
   int32_t* bc = nullptr;
   int32_t* wc = nullptr;
+  int32_t* raw_len_d = nullptr;     // raw lengths (cutoffs are computed from them on the device)
   float t_loaded = -1.0f;
   // per-job round state (allocated with the store)
   int32_t* cluster = nullptr;
@@ -130,9 +134,16 @@ hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, con
                                int fork = 1);
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
-int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
+// fixed != nullptr: run in that workspace (the no-allocation entry's
+// scratch) instead of the context's, without events or graphs.
+int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats,
+                  const Workspace* fixed = nullptr);
 int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
-                    int32_t* const* out, gsdr_stats* stats);
+                    int32_t* const* out, gsdr_stats* stats, const Workspace* fixed = nullptr);
+// No-allocation entry (noalloc.hip): the workspace lives in a file-scope
+// device array whose per-device address scratch_setup looks up once.
+int scratch_setup(int device, void** ptr);
+int cluster_noalloc(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
 int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_params* params,
                  int32_t* const* out, gsdr_stats* stats);
 int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq, const int32_t* pairs,
@@ -157,8 +168,8 @@ struct StepLaunch {
   int reps_grid[kNumShapes] = {};
   int reps_filter_grid = 1;
 };
-StepLaunch make_step_launch(gsdr_ctx* c, int njobs, int smax, unsigned kmask, int32_t* jobs_left,
-                            const Tuning& T = Tuning());
+StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
+                            int32_t* jobs_left, const Tuning& T = Tuning());
 // Alignment workgroups per CU for a tile shape under knobs T; 0 = the
 // shape kernel's resident blocks per CU (registers and LDS decide it).
 // Measured with 8 per CU against 4: many short pairs gain (200,000 records,

@@ -216,6 +216,31 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
   }
 }
 
+// ----------------------------------------------------------------- cutoffs
+// createCutoff (func.cpp:339-374) in float32, operation for operation like
+// the reference: bc = ceilf((float)L * t) (func.cpp:368-369) and
+// wc = max(L-k+1 - (int)(ceilf((float)L * (1.0f-t)) * k), 1) with the offset
+// factor forced to 1 (func.cpp:353-366). Each product and difference is
+// rounded on its own (no contraction), so the values equal the host's.
+__global__ __launch_bounds__(256) void cutoff_kernel(const int32_t* __restrict__ raw_len, int n, float t, int k,
+                                                     int32_t* __restrict__ bc, int32_t* __restrict__ wc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int L = raw_len[i];
+  const float fl = (float)L;
+  bc[i] = (int)ceilf(__fmul_rn(fl, t));
+  const float ce = ceilf(__fmul_rn(fl, __fsub_rn(1.0f, t)));
+  const int cut = (int)__fmul_rn(ce, (float)k);
+  const int req = L - k + 1 - cut;
+  wc[i] = req < 1 ? 1 : req;
+}
+
+hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cutoff_kernel, dim3((n + 255) / 256), dim3(256), 0, s, raw_len, n, t, k, bc, wc);
+  return hipGetLastError();
+}
+
 // -------------------------------------------------------------- launchers
 template <int G>
 static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipStream_t s) {

@@ -1185,6 +1185,8 @@ int align_blocks_per_cu(int shape);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s);
+// Cutoffs of n sequences for threshold t and word length k (func.cpp:339-374).
+hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
                             int32_t* jobs_left, hipStream_t s);
 

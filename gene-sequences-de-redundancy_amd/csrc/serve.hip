@@ -204,7 +204,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
     // recapture when the table changed, or when another call on the context
     // reallocated the workspace the graph's kernels point into
     if (!srv->exec || srv->graph_jobs != srv->running || srv->graph_ws != c->ws.generation) {
-      const StepLaunch L = make_step_launch(c, nj, smax, kmask, nullptr);
+      const StepLaunch L = make_step_launch(c, c->ws, nj, smax, kmask, nullptr);
       if ((e = capture_steps(srv, L, q)) != hipSuccess) return hip_fail(e, "serve capture");
     }
     if ((e = hipGraphLaunch(srv->exec, q)) != hipSuccess) return hip_fail(e, "serve steps");

@@ -85,10 +85,20 @@ class Tuning(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved2"}
 
 
+class Options(C.Structure):
+    """gsdr_options: the reference's Option (func.h:11-16) plus long options."""
+    _fields_ = [("input", C.c_char_p), ("output", C.c_char_p), ("threshold", C.c_float), ("word_length", C.c_int32),
+                ("mode", C.c_int32), ("spec_reps", C.c_int32), ("device", C.c_int32), ("drop_filters", C.c_int32),
+                ("membership", C.c_char_p), ("report", C.c_char_p)]
+
+
 # every entry point of include/gsdr.h: (name, restype, argtypes)
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 SIGNATURES = [
+    ("gsdr_parse_options", C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Options)]),
+    ("gsdr_cutoffs", C.c_int, [C.c_float, C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32)]),
     ("gsdr_records_read_fasta", C.c_int, [C.c_char_p, _PP]),
     ("gsdr_records_read_fasta_sorted", C.c_int, [C.c_char_p, _PP]),
     ("gsdr_records_from_buffers", C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.c_char_p,
@@ -248,6 +258,28 @@ class Records:
             self.free()
         except Exception:
             pass
+
+
+def parse_options(argv):
+    """checkOption (func.cpp:6-57) through the C ABI: argv without the
+    program name. Returns an Options copy, or raises GsdrError with the
+    reference's usage text."""
+    args = [b"cluster"] + [a if isinstance(a, bytes) else a.encode() for a in argv]
+    arr = (C.c_char_p * len(args))(*args)
+    o = Options()
+    _check(lib().gsdr_parse_options(len(args), arr, C.byref(o)), "parse_options")
+    return {"input": o.input, "output": o.output, "threshold": o.threshold, "word_length": o.word_length,
+            "mode": o.mode, "spec_reps": o.spec_reps, "device": o.device, "drop_filters": o.drop_filters,
+            "membership": o.membership, "report": o.report}
+
+
+def cutoffs(raw_lengths, threshold, word_length=0):
+    """createCutoff (func.cpp:339-374): (bc, wc) arrays for raw lengths."""
+    L = np.ascontiguousarray(raw_lengths, dtype=np.int32)
+    bc = np.zeros(len(L), dtype=np.int32)
+    wc = np.zeros(len(L), dtype=np.int32)
+    _check(lib().gsdr_cutoffs(threshold, word_length, _i32p(L), len(L), _i32p(bc), _i32p(wc)), "cutoffs")
+    return bc, wc
 
 
 def auto_word_length(t):

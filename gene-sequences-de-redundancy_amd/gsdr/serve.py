@@ -85,6 +85,7 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
     results = {}
     error = []
     nxt = [0]
+    max_running = 0
     t0 = time.perf_counter()
 
     def loader():
@@ -115,6 +116,7 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
         if error:
             raise error[0]
         running = srv.step()
+        max_running = max(max_running, running)
         with lock:
             pending = list(submitted.items())
         for jid, (i, t_arr) in pending:
@@ -138,5 +140,5 @@ def run_stream(engine, specs, records, rate, hbm_budget, max_slots, seed=0, step
     lat = np.array([r["latency_s"] for r in out])
     summary = {"jobs": len(out), "rate_jobs_per_s": rate, "wall_s": wall,
                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-               "sustained_mbp_per_s": sum(r["bases"] for r in out) / 1e6 / wall}
+               "sustained_mbp_per_s": sum(r["bases"] for r in out) / 1e6 / wall, "max_concurrent": max_running}
     return out, summary

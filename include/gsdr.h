@@ -100,6 +100,37 @@ typedef struct gsdr_tuning {
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
 } gsdr_tuning;
 
+/* ---- options (host side): replaces checkOption (func.cpp:6-57) and the
+ *      Option struct (func.h:11-16) ---- */
+typedef struct gsdr_options {
+  const char *input;      /* 'i': default "testData.fasta" (func.cpp:8) */
+  const char *output;     /* 'o': default "result.fasta" (func.cpp:9) */
+  float threshold;        /* 't': std::stof, default 0.95, 0.8 <= t < 1 (func.cpp:10,21,31-35) */
+  int32_t word_length;    /* 'w': 4..8, or from t when not given (func.cpp:11,36-51) */
+  /* long options of this build (not in the reference) */
+  int32_t mode;           /* --mode plain|batched (default batched) */
+  int32_t spec_reps;      /* --spec K */
+  int32_t device;         /* --device N */
+  int32_t drop_filters;   /* --drop 0|1 */
+  const char *membership; /* --membership FILE, or NULL */
+  const char *report;     /* --report FILE, or NULL */
+} gsdr_options;
+
+/* Parses the reference's argv letter/value pairs (`t 0.9 i in.fasta o
+ * out.fasta w 6`; `-t` etc. and the long options above are accepted too).
+ * Strings point into argv or to static defaults. Where the reference prints
+ * its usage text and exits with status 0 (odd argument count, unknown
+ * letter, t or w out of range), this returns GSDR_E_INVALID and
+ * gsdr_last_error() holds the reference's message lines; the caller prints
+ * them and exits 0 to keep the reference's behaviour. */
+int gsdr_parse_options(int argc, char **argv, gsdr_options *out);
+
+/* Cutoffs of n sequences of raw lengths raw_len (createCutoff,
+ * func.cpp:339-374), in float32 like the reference: bc = ceilf(L*t) and
+ * wc = max(L-k+1 - (int)(ceilf(L*(1-t))*k), 1). word_length 0 = from t. */
+int gsdr_cutoffs(float threshold, int32_t word_length, const int32_t *raw_len, int64_t n, int32_t *bc_out,
+                 int32_t *wc_out);
+
 /* ---- records (host side): replaces readFile/saveFile (func.cpp:70-112,
  *      785-799) and the Read vector (func.h:17-20) ---- */
 

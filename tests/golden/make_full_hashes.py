@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/full_hashes.json: oracle answers at full size.
+
+For every workload below, the seeded generator of the library writes the
+FASTA input (80-column lines), the CPU oracle (oracle/build/liborc.so, exact
+fast mode: band + early exit, checked against the full LCS by
+tests/test_oracle.py) clusters it, and the fixture keeps sha256 digests of
+
+  fasta    the generated FASTA file (pins the input itself),
+  cluster  cluster[] over sorted indices as int32 little endian,
+  result   the result.fasta the oracle writes (func.cpp:785-796),
+
+plus the representative count. The GPU tests regenerate each input, check
+its digest, cluster it through the C ABI and compare the other two.
+
+Workloads:
+  single   10,000-record families at t=0.95; 200,000 log-normal records at
+           t=0.85, 0.90, 0.95; 2,000,000 Pareto records at t=0.95 and 0.80
+  many     the first 64 jobs of the 512-job workload (sizes log-uniform in
+           [500, 50,000], medians {150, 400, 1500, 5000}, t in
+           {.80, .85, .90, .95, .99})
+  stream   the first 64 jobs of the serving stream (sizes log-uniform in
+           [1,000, 200,000], t in {.80, .90, .95, .99})
+
+Run in the development container (takes tens of minutes on 8 cores):
+  python tests/golden/make_full_hashes.py [--only NAME ...] [--threads N]
+Entries already in the file are kept unless --redo is given.
+"""
+import argparse
+import concurrent.futures as cf
+import hashlib
+import json
+import os
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "gene-sequences-de-redundancy_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import gsdr  # noqa: E402
+import oracle_ffi as O  # noqa: E402
+from gsdr import workloads as W  # noqa: E402
+
+OUT = os.path.join(HERE, "full_hashes.json")
+
+SINGLE_RUNS = [("c1", 0.95), ("c2", 0.85), ("c2", 0.90), ("c2", 0.95), ("c4", 0.95), ("c4", 0.80)]
+MANY_COUNT = 64
+STREAM_COUNT = 64
+
+
+def sha_file(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def sha_cluster(cl):
+    return hashlib.sha256(np.ascontiguousarray(cl, dtype="<i4").tobytes()).hexdigest()
+
+
+def oracle_entry(spec, t, threads, tmp):
+    """Generates the spec's FASTA, clusters it with the oracle, returns the digests."""
+    tag = f"{spec['law']}_{spec['seed']}_{spec['n']}_{t}"
+    fa = os.path.join(tmp, tag + ".fasta")
+    res = os.path.join(tmp, tag + ".result.fasta")
+    recs = W.generate(spec)
+    recs.write_fasta(fa)
+    bases = recs.bases
+    recs.free()
+    t0 = time.perf_counter()
+    cl, _, st = O.cluster_file(fa, t, threads=threads, fast=True, out_path=res)
+    dt = time.perf_counter() - t0
+    out = {"law": spec["law"], "seed": spec["seed"], "n": spec["n"], "median": spec.get("median", 0.0), "t": t,
+           "bases": int(bases), "reps": int(st["reps"]), "fasta": sha_file(fa), "cluster": sha_cluster(cl),
+           "result": sha_file(res), "oracle_s": round(dt, 2)}
+    os.remove(fa)
+    os.remove(res)
+    return out
+
+
+def load():
+    if os.path.exists(OUT):
+        with open(OUT) as f:
+            return json.load(f)
+    return {"note": "oracle digests at full size; written by make_full_hashes.py", "single": {}, "many": [],
+            "stream": []}
+
+
+def save(d):
+    tmp = OUT + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+        f.write("\n")
+    os.replace(tmp, OUT)
+
+
+def run_jobs(specs, have, threads, tmp, label, keep):
+    """Oracle runs of many small jobs, several at once (one thread each);
+    keep(list) stores the entries finished so far."""
+    done = {(e["seed"], e["t"]): e for e in have}
+    todo = [s for s in specs if (s["seed"], s["t"]) not in done]
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = {ex.submit(oracle_entry, s, s["t"], 1, tmp): s for s in todo}
+        for f in cf.as_completed(futs):
+            e = f.result()
+            done[(e["seed"], e["t"])] = e
+            keep([done[(s["seed"], s["t"])] for s in specs if (s["seed"], s["t"]) in done])
+            print(f"{label} job n={e['n']} t={e['t']}: reps {e['reps']}, {e['oracle_s']} s", flush=True)
+    return [done[(s["seed"], s["t"])] for s in specs]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*", default=[], help="c1_0.95 ... many stream")
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
+    ap.add_argument("--redo", action="store_true")
+    args = ap.parse_args()
+    d = load()
+    want = set(args.only)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, t in SINGLE_RUNS:
+            key = f"{name}_{t:.2f}"
+            if want and key not in want:
+                continue
+            if key in d["single"] and not args.redo:
+                continue
+            law, seed, n, _, _ = W.SINGLE[name]
+            e = oracle_entry({"law": law, "seed": seed, "n": n}, t, args.threads, tmp)
+            d["single"][key] = e
+            save(d)
+            print(f"{key}: reps {e['reps']}, oracle {e['oracle_s']} s", flush=True)
+        if not want or "many" in want:
+            specs = W.many_jobs(MANY_COUNT)
+            d["many"] = run_jobs(specs, [] if args.redo else d.get("many", []), args.threads, tmp, "many",
+                                 lambda v: (d.__setitem__("many", v), save(d)))
+            save(d)
+        if not want or "stream" in want:
+            specs = W.stream_jobs(STREAM_COUNT)
+            d["stream"] = run_jobs(specs, [] if args.redo else d.get("stream", []), args.threads, tmp, "stream",
+                                   lambda v: (d.__setitem__("stream", v), save(d)))
+            save(d)
+
+
+if __name__ == "__main__":
+    main()

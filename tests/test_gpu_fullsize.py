@@ -1,0 +1,186 @@
+"""Full-size parity: the HIP path against the oracle's answers at the stated sizes.
+
+tests/golden/full_hashes.json holds, per workload, sha256 digests of the
+generated FASTA, of the oracle's cluster[] (int32 little endian, sorted
+order) and of the oracle's result.fasta (written in the development
+container by tests/golden/make_full_hashes.py). Each test regenerates the
+input with the library's seeded generator, checks the input digest, clusters
+through the C ABI and compares the other two digests:
+
+  single jobs   10,000 families t=0.95; 200,000 log-normal records at
+                t=0.85/0.90/0.95; 2,000,000 Pareto records at t=0.95/0.80,
+                in the batched and the plain form
+  many jobs     64 jobs of the 512-job workload in one gsdr_cluster_jobs call
+  stream        64 jobs of the serving stream through the continuous-batching
+                loop (Poisson arrivals, admission by slot and HBM budget)
+  drop-in       the `cluster` binary (result.fasta, --membership) and the
+                reference-side binding of INTEGRATION.md, run as programs
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gsdr
+import oracle_ffi as O
+from gsdr import serve
+from gsdr import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = json.load(open(os.path.join(ROOT, "tests", "golden", "full_hashes.json")))
+CLI = os.path.join(ROOT, "gene-sequences-de-redundancy_amd", "build", "cluster")
+
+
+def sha_file(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def sha_cluster(cl):
+    return hashlib.sha256(np.ascontiguousarray(cl, dtype="<i4").tobytes()).hexdigest()
+
+
+_inputs = {}
+
+
+def fasta_for(entry, tmp_path_factory):
+    """The entry's input, generated once per session; its digest must match."""
+    key = (entry["law"], entry["seed"], entry["n"], entry.get("median", 0.0))
+    if key not in _inputs:
+        d = tmp_path_factory.mktemp("fullsize")
+        fa = str(d / f"in_{entry['law']}_{entry['seed']}_{entry['n']}.fasta")
+        r = W.generate(entry)
+        r.write_fasta(fa)
+        r.free()
+        assert sha_file(fa) == entry["fasta"], "generated input differs from the fixture's"
+        _inputs[key] = fa
+    return _inputs[key]
+
+
+SINGLE = sorted(FIX["single"])
+
+
+@pytest.mark.parametrize("mode", [gsdr.MODE_BATCHED, gsdr.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("key", SINGLE)
+def test_single_job_matches_oracle_digests(engine, tmp_path, tmp_path_factory, key, mode):
+    e = FIX["single"][key]
+    fa = fasta_for(e, tmp_path_factory)
+    print(f"\n# full size {key} mode={mode} n={e['n']}", flush=True)
+    r = gsdr.Records.read_fasta(fa, length_order=True)
+    r.sort()
+    try:
+        cl, st = gsdr.cluster_records(engine, r, e["t"], mode=mode)
+        assert st.reps == e["reps"]
+        assert sha_cluster(cl) == e["cluster"], f"{key}: cluster[] differs from the oracle's"
+        res = str(tmp_path / "result.fasta")
+        r.write_result(cl, res)
+        assert sha_file(res) == e["result"], f"{key}: result.fasta differs from the oracle's"
+        # every considered pair is rejected by exactly one stage or aligned
+        assert st.pairs == st.rej_length + st.rej_comp + st.rej_word + st.aligned
+        assert st.aligned == st.accepted + st.rej_align
+    finally:
+        r.free()
+
+
+def test_many_jobs_in_one_call_match_oracle_digests(engine, tmp_path):
+    """64 jobs of the 512-job workload (per-job medians 150..5,000 and t
+    0.80..0.99) clustered together in one batched call; every job's
+    cluster[] and result.fasta against the oracle's."""
+    jobs = FIX["many"]
+    if len(jobs) != 64:
+        pytest.skip("the oracle digests of the 64-job sample are not in the fixture yet")
+    recs, stores = [], []
+    try:
+        for e in jobs:
+            r = W.generate(e)
+            r.sort()
+            recs.append(r)
+            stores.append(engine.load(r, gsdr.auto_word_length(e["t"])))
+        outs, st = engine.cluster_jobs(stores, [e["t"] for e in jobs], mode=gsdr.MODE_BATCHED)
+        bad = [i for i, (e, cl) in enumerate(zip(jobs, outs)) if sha_cluster(cl) != e["cluster"]]
+        assert not bad, f"jobs {bad} differ from the oracle"
+        for i in range(0, len(jobs), 8):
+            res = str(tmp_path / f"r{i}.fasta")
+            recs[i].write_result(outs[i], res)
+            assert sha_file(res) == jobs[i]["result"]
+        assert st.reps == sum(e["reps"] for e in jobs)
+    finally:
+        for s in stores:
+            s.free()
+        for r in recs:
+            r.free()
+
+
+def test_stream_jobs_match_oracle_digests(engine):
+    """64 jobs of the serving stream (sizes 1,000..200,000, t in
+    {.80,.90,.95,.99}) submitted at Poisson arrival times into the
+    continuous-batching server with fewer slots than jobs, so later jobs are
+    admitted while earlier ones run; every job's cluster[] against the
+    oracle's."""
+    jobs = FIX["stream"]
+    if len(jobs) != 64:
+        pytest.skip("the oracle digests of the 64-job stream sample are not in the fixture yet")
+    specs = [{"id": i, "n": e["n"], "t": e["t"], "seed": e["seed"], "law": 2, "median": e["median"]}
+             for i, e in enumerate(jobs)]
+    recs = [W.generate(s, sort=True) for s in specs]
+    try:
+        out, summary = serve.run_stream(engine, specs, recs, rate=40.0, hbm_budget=24 << 30, max_slots=12, seed=7)
+        got = {r["job"]: r["cluster"] for r in out}
+        bad = [i for i, e in enumerate(jobs) if sha_cluster(got[i]) != e["cluster"]]
+        assert not bad, f"jobs {bad} differ from the oracle"
+        assert summary["jobs"] == 64 and summary["max_concurrent"] > 1
+    finally:
+        for r in recs:
+            r.free()
+
+
+def check_membership(path, fa, cluster_digest):
+    """Columns of the membership file against the oracle: sorted index,
+    original index (the oracle's length order), representative (the oracle's
+    cluster[] digest) and name (the oracle's reader)."""
+    rows = [line.rstrip(b"\n").split(b"\t", 3) for line in open(path, "rb")]
+    reads = O.read_fasta(fa)
+    perm = O.sort_order(np.array([len(d) for _, d in reads], dtype=np.int64))
+    assert len(rows) == len(reads)
+    assert all(int(r[0]) == s for s, r in enumerate(rows))
+    assert np.array_equal(np.array([int(r[1]) for r in rows], dtype=np.int64), perm)
+    assert sha_cluster(np.array([int(r[2]) for r in rows], dtype=np.int32)) == cluster_digest
+    assert all(r[3] == reads[perm[s]][0] for s, r in enumerate(rows))
+
+
+@pytest.mark.parametrize("key,mode", [("c1_0.95", "plain"), ("c2_0.90", "batched")])
+def test_cli_drop_in_matches_oracle(tmp_path, tmp_path_factory, key, mode):
+    """The `cluster` binary with the reference's letter options writes the
+    oracle's result.fasta, and its membership file holds the oracle's
+    assignments."""
+    e = FIX["single"][key]
+    fa = fasta_for(e, tmp_path_factory)
+    out, mem = str(tmp_path / "out.fasta"), str(tmp_path / "m.tsv")
+    r = subprocess.run([CLI, "t", str(e["t"]), "i", fa, "o", out, "--membership", mem, "--mode", mode],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert f"cluster：{e['reps']}" in r.stdout
+    assert sha_file(out) == e["result"]
+    check_membership(mem, fa, e["cluster"])
+
+
+def test_integration_binding_runs(tmp_path, tmp_path_factory):
+    """INTEGRATION.md's reference-side main.cpp, built against libgsdr.so
+    and run on the 10,000-record input, writes the oracle's result.fasta."""
+    from test_host import build_integration_example
+
+    e = FIX["single"]["c1_0.95"]
+    fa = fasta_for(e, tmp_path_factory)
+    exe = build_integration_example(tmp_path)
+    out = str(tmp_path / "abi.fasta")
+    r = subprocess.run([str(exe), "i", fa, "o", out, "t", str(e["t"])], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert sha_file(out) == e["result"]

@@ -173,3 +173,79 @@ def test_reference_result_subset_full_lcs(tmp_path):
     fa = os.path.join(GOLDEN, "result_subset.fasta")
     cl, _, _ = O.cluster_file(fa, 0.95, fast=False)
     assert (cl == np.arange(len(cl))).all()
+
+
+# thresholds of the workloads plus the two k-map boundaries (float vs double)
+CUTOFF_THRESHOLDS = (0.80, 0.85, 0.88, 0.90, 0.94, 0.95, 0.99)
+
+
+def test_product_cutoffs_match_oracle_table():
+    """The product's float32 cutoffs (createCutoff, func.cpp:339-374) over
+    every raw length 0..32,768 at every threshold and word length, against
+    the oracle's independently written expressions (which tests/test_oracle.py
+    pins to exact rational arithmetic and the float32 exception sets)."""
+    lengths = np.arange(0, 32769, dtype=np.int32)
+    for t in CUTOFF_THRESHOLDS:
+        for k in (0, 4, 5, 6, 7, 8):
+            bc, wc = gsdr.cutoffs(lengths, t, k)
+            kk = k or O.kmap(t)
+            want = np.array([O.cutoffs(int(L), t, kk) for L in lengths], dtype=np.int32)
+            assert np.array_equal(bc, want[:, 0]), (t, k)
+            assert np.array_equal(wc, want[:, 1]), (t, k)
+    # the float32 quirk: ceilf(100 * (1 - 0.95f)) is 6, so wc(100) = 95 - 36
+    bc, wc = gsdr.cutoffs([100], 0.95)
+    assert (bc[0], wc[0]) == (95, 59)
+
+
+def test_parse_options_follows_check_option():
+    """checkOption (func.cpp:6-57) through the ABI: defaults, letters, the
+    dashed forms, k from a float threshold compared with double literals,
+    and the reference's usage/range messages instead of exit(0)."""
+    d = gsdr.parse_options([])
+    assert (d["input"], d["output"], d["word_length"]) == (b"testData.fasta", b"result.fasta", 6)
+    assert abs(d["threshold"] - 0.95) < 1e-7 and d["mode"] == gsdr.MODE_BATCHED
+    o = gsdr.parse_options(["t", "0.9", "i", "in.fa", "o", "out.fa", "w", "8"])
+    assert (o["input"], o["output"], o["word_length"]) == (b"in.fa", b"out.fa", 8)
+    assert gsdr.parse_options(["-t", "0.88"])["word_length"] == 4  # 0.88f < 0.88 (double)
+    assert gsdr.parse_options(["t", "0.94"])["word_length"] == 5
+    assert gsdr.parse_options(["t", "0.97"])["word_length"] == 7
+    assert gsdr.parse_options(["input", "x"])["input"] == b"x"  # the first letter decides (func.cpp:13)
+    o = gsdr.parse_options(["--mode", "plain", "--membership", "m.tsv", "--drop", "1", "t", "0.8"])
+    assert (o["mode"], o["membership"], o["drop_filters"], o["word_length"]) == (gsdr.MODE_PLAIN, b"m.tsv", 1, 4)
+    usage = "use like this: cluster i inputFile t threshold"
+    for argv, first in ((["t"], usage), (["x", "1"], usage), (["t", "0.7"], "similarity out of range"),
+                        (["t", "1.0"], "similarity out of range"), (["w", "9"], "word length out of range"),
+                        (["w", "3"], "word length out of range"), (["t", "abc"], usage),
+                        (["--mode", "fast"], usage)):
+        with pytest.raises(gsdr.GsdrError) as e:
+            gsdr.parse_options(argv)
+        assert e.value.status == gsdr.E_INVALID
+        msg = gsdr.lib().gsdr_last_error().decode()
+        assert msg.startswith(first) and msg.endswith(usage), (argv, msg)
+
+
+def integration_example():
+    """The C++ block of INTEGRATION.md section 2 (the reference-side binding)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    sec = text[text.index("## 2."):text.index("## 3.")]
+    return sec[sec.index("```cpp") + len("```cpp"):sec.index("```", sec.index("```cpp") + 6)]
+
+
+def build_integration_example(tmp_path):
+    src = tmp_path / "main.cpp"
+    src.write_text(integration_example())
+    exe = tmp_path / "cluster_abi"
+    libdir = os.path.dirname(gsdr.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", str(src), "-I", os.path.join(ROOT, "include"),
+                    "-L", libdir, "-lgsdr", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_integration_example_builds(tmp_path):
+    """The maintainer's binding compiles and links against libgsdr.so with
+    only include/gsdr.h (no func.h, no SYCL), and keeps the reference's
+    usage behaviour: message on stdout, exit status 0."""
+    exe = build_integration_example(tmp_path)
+    for argv, want in ((["t", "0.5"], "similarity out of range"), (["t"], "use like this")):
+        r = subprocess.run([str(exe)] + argv, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and want in r.stdout, (argv, r.stdout, r.stderr)
