@@ -267,6 +267,9 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
     HIP_TRY(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming), "hipEventCreate");
   }
   HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate");
+  // this device's copy of the no-allocation scratch array (device current)
+  const int sst = scratch_setup(device, &c->scratch);
+  if (sst != GSDR_OK) return sst;
   *out = c.release();
   return GSDR_OK;
 }
@@ -323,6 +326,13 @@ int64_t gsdr_store_footprint(const gsdr_store* s) { return s ? s->footprint : -1
 int gsdr_cluster(gsdr_ctx* c, gsdr_store* s, const gsdr_params* p, int32_t* cluster_out, gsdr_stats* stats) {
   if (!c || !s || !p || !cluster_out) return fail(GSDR_E_INVALID, "null argument");
   return cluster_store(c, *s, *p, cluster_out, stats);
+}
+
+int gsdr_cluster_noalloc(gsdr_ctx* c, gsdr_store* s, const gsdr_params* p, int32_t* cluster_out,
+                         gsdr_stats* stats) {
+  if (!c || !s || !p || !cluster_out) return fail(GSDR_E_INVALID, "null argument");
+  if (s->ctx != c) return fail(GSDR_E_INVALID, "store belongs to another context");
+  return cluster_noalloc(c, *s, *p, cluster_out, stats);
 }
 
 int gsdr_cluster_jobs(gsdr_ctx* c, int32_t njobs, gsdr_store* const* stores, const gsdr_params* params,

@@ -120,6 +120,7 @@ SIGNATURES = [
     ("gsdr_store_free", None, [_P]),
     ("gsdr_store_footprint", C.c_int64, [_P]),
     ("gsdr_cluster", C.c_int, [_P, _P, C.POINTER(Params), C.POINTER(C.c_int32), C.POINTER(Stats)]),
+    ("gsdr_cluster_noalloc", C.c_int, [_P, _P, C.POINTER(Params), C.POINTER(C.c_int32), C.POINTER(Stats)]),
     ("gsdr_cluster_jobs", C.c_int, [_P, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(Params),
                                     C.POINTER(C.POINTER(C.c_int32)), C.POINTER(Stats)]),
     ("gsdr_autotune", C.c_int, [_P, _P, C.POINTER(Params), C.c_int32, C.POINTER(Tuning)]),
@@ -372,7 +373,9 @@ class Store:
         return lib().gsdr_store_footprint(self.h)
 
     def cluster(self, threshold, word_length=0, mode=MODE_PLAIN, spec_reps=0, timing=False, tune=False,
-                drop_filters=False):
+                drop_filters=False, noalloc=False):
+        """gsdr_cluster, or with noalloc gsdr_cluster_noalloc (the launch
+        workspace in the library's fixed device array, no allocation)."""
         p = Params()
         p.drop_filters = 1 if drop_filters else 0
         p.threshold = threshold
@@ -383,7 +386,8 @@ class Store:
         p.tune = 1 if tune else 0
         out = np.zeros(self.n, dtype=np.int32)
         st = Stats()
-        _check(lib().gsdr_cluster(self.engine.h, self.h, C.byref(p), _i32p(out), C.byref(st)), "cluster")
+        fn = lib().gsdr_cluster_noalloc if noalloc else lib().gsdr_cluster
+        _check(fn(self.engine.h, self.h, C.byref(p), _i32p(out), C.byref(st)), "cluster")
         return out, st
 
     def autotune(self, threshold, word_length=0, spec_reps=0, max_trials=0):

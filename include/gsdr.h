@@ -192,6 +192,21 @@ int64_t gsdr_store_footprint(const gsdr_store *store);
 int gsdr_cluster(gsdr_ctx *ctx, gsdr_store *store, const gsdr_params *params,
                  int32_t *cluster_out, gsdr_stats *stats);
 
+/* No counterpart in the reference. gsdr_cluster for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same results, both
+ * forms. The launch workspace (alignment lists, counters, job table) lives in
+ * a fixed device array of GSDR_NOALLOC_SCRATCH_BYTES per GPU, held from the
+ * first gsdr_open on that GPU until the process ends; the call itself
+ * allocates nothing (no hipMalloc, hipHostMalloc, events or graphs). Limits:
+ * at most GSDR_NOALLOC_MAX_SEQS sequences (GSDR_E_LIMIT beyond), no timing
+ * (params.timing must be 0). Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (several contexts or threads) race. */
+#define GSDR_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define GSDR_NOALLOC_MAX_SEQS 600000
+int gsdr_cluster_noalloc(gsdr_ctx *ctx, gsdr_store *store, const gsdr_params *params, int32_t *cluster_out,
+                         gsdr_stats *stats);
+
 /* Many independent jobs in shared persistent launches (no reference
  * counterpart; each job's result equals gsdr_cluster on that job alone). */
 int gsdr_cluster_jobs(gsdr_ctx *ctx, int32_t njobs, gsdr_store *const *stores,
