@@ -46,8 +46,11 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
   if ((st = dev_alloc(A, F, &s.bc, (size_t)n)) || (st = dev_alloc(A, F, &s.wc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
-      (st = dev_alloc(A, F, &s.state, (size_t)1)) || (st = dev_alloc(A, F, &s.raw_len_d, (size_t)n)))
+      (st = dev_alloc(A, F, &s.state, (size_t)1)) || (st = dev_alloc(A, F, &s.raw_len_d, (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.ksum, (size_t)n)))
     return st;
+  HIP_TRY(launch_profile_totals(s.kp, s.kpoff, (int)n, s.ksum, c->upload), "profile totals");
+  HIP_TRY(hipStreamSynchronize(c->upload), "profile totals");
   HIP_TRY(hipMemcpy(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice),
           "upload lengths");
   return GSDR_OK;

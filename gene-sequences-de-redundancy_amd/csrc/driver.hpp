@@ -81,6 +81,7 @@ struct gsdr_store {
   int32_t* bc = nullptr;
   int32_t* wc = nullptr;
   int32_t* raw_len_d = nullptr;     // raw lengths (cutoffs are computed from them on the device)
+  int32_t* ksum = nullptr;          // per sequence: sum of its stored k-mer counts
   float t_loaded = -1.0f;
   // per-job round state (allocated with the store)
   int32_t* cluster = nullptr;
@@ -93,6 +94,7 @@ struct gsdr_store {
   gsdr::StoreView view() const {
     gsdr::StoreView v;
     v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
+    v.ksum = ksum;
     return v;
   }
   gsdr::JobDev job(const gsdr_params& p) const {

@@ -216,6 +216,29 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
   }
 }
 
+// ---------------------------------------------------------- profile totals
+// One wave per sequence: T_s = sum of the counts (modulo 2^16, as stored) of
+// its distinct k-mers. The shared-word sum with any representative is at most
+// what was summed so far plus the counts not yet read, which bounds the scan
+// of the one-table filter.
+__global__ __launch_bounds__(256) void profile_total_kernel(const uint32_t* __restrict__ kp,
+                                                            const int64_t* __restrict__ kpoff, int n,
+                                                            int32_t* __restrict__ ksum) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n) return;
+  const int lane = threadIdx.x & 63;
+  int t = 0;
+  for (int64_t e = kpoff[s] + lane; e < kpoff[s + 1]; e += 64) t += (int)(kp[e] >> 16);
+  t = wave_sum(t);
+  if (lane == 0) ksum[s] = t;
+}
+
+hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n, int32_t* ksum, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(profile_total_kernel, dim3((n + 3) / 4), dim3(256), 0, s, kp, kpoff, n, ksum);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- cutoffs
 // createCutoff (func.cpp:339-374) in float32, operation for operation like
 // the reference: bc = ceilf((float)L * t) (func.cpp:368-369) and
@@ -244,7 +267,9 @@ hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t
 // -------------------------------------------------------------- launchers
 template <int G>
 static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
-  const size_t lds = (size_t)G * (sizeof(uint16_t) << (2 * max_k));
+  constexpr int ts = filter_table_stride(G);
+  if (G > 1 && (1 << (2 * max_k)) > ts) return hipErrorInvalidValue;  // the tables would overlap
+  const size_t lds = G > 1 ? (size_t)G * ts * sizeof(uint16_t) : (sizeof(uint16_t) << (2 * max_k));
   static bool attr_set = false;
   if (lds > 32768 && !attr_set) {
     hipError_t e = hipFuncSetAttribute((const void*)filter_kernel<G>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -77,6 +77,7 @@ struct StoreView {
   const int64_t* kpoff;    // n+1 offsets into kp
   const int32_t* bc;       // ceil(L*t) (func.cpp:368)
   const int32_t* wc;       // shared-word cutoff (func.cpp:353-366)
+  const int32_t* ksum;     // sum of the stored k-mer counts (the profile's total)
 };
 
 // Speculative representatives per step are bounded by the accept-mask width.

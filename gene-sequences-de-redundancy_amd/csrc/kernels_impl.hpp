@@ -106,9 +106,16 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
 // widest group is held to that too.
 __host__ __device__ constexpr int filter_waves_per_eu(int g) { return g >= 4 ? 4 : 1; }
 
+// Entries per representative table in LDS: a compile-time stride, so every
+// lookup addresses its table with an immediate offset (32 KiB for the group;
+// 4^k must fit, which filter_group guarantees: G = 16 for k <= 5, 4 for k = 6;
+// one table of 4^k entries for k >= 7).
+__host__ __device__ constexpr int filter_table_stride(int g) { return g > 1 ? 16384 / g : 0; }
+
 template <int G>
 __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(FilterArgs a) {
   static_assert(G >= 1 && G <= 16, "one lane of a 16-lane group per representative");
+  constexpr int kTS = filter_table_stride(G);
   HIP_DYNAMIC_SHARED(uint32_t, lds)
   uint16_t* table = reinterpret_cast<uint16_t*>(lds);
   __shared__ int s_pre[kMaxJobsPerLaunch + 1];
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       const int nw = 1 << (2 * J.k);
       gcount = min(G, members - gi * G);
       __syncthreads();
-      for (int i = tid; i < G * nw / 2; i += 256) lds[i] = 0u;
+      for (int i = tid; i < (G > 1 ? G * kTS : nw) / 2; i += 256) lds[i] = 0u;
       if (tid < gcount) {
         const int rank = gi * G + tid;
         int slot = rank;
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       for (int g = 0; g < gcount; ++g) {
         const int r = s_rep[g];
         const int64_t k0 = J.st.kpoff[r], k1 = J.st.kpoff[r + 1];
-        uint16_t* tg = table + g * nw;
+        uint16_t* tg = table + g * kTS;
         for (int64_t e = k0 + tid; e < k1; e += 256) {
           const uint32_t x = J.st.kp[e];
           tg[x & 0xFFFFu] = (uint16_t)(x >> 16);
@@ -247,35 +254,86 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       // shared words: q's profile read once, one table lookup per survivor so
       // far (none when no representative is left; drop mode reads no profile
       // and passes every pair)
-      const int nw = 1 << (2 * J.k);
       const int64_t e0 = J.st.kpoff[q], e1 = (drop || mask == 0) ? e0 : J.st.kpoff[q + 1];
       int acc[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) acc[g] = 0;
-      // the profile loads are latency bound: NB (16, or 8 beside 16 tables,
-      // which need the registers), then 4, in flight per lane
-      constexpr int NB = G >= 16 ? 8 : 16;
+      // the profile loads are latency bound: NB, then 4, in flight per lane.
+      // With 16 tables every lookup runs, whether or not its representative
+      // is still in play (its total is discarded below): the per-table
+      // branches would hold 16 divergent exec masks in scalar registers.
+      const int wcq = drop ? 0 : J.st.wc[q];
       int64_t e = e0 + gl;
-      for (; e + (NB - 1) * 16 < e1; e += NB * 16) {
-        uint32_t xs[NB];
+      bool cut = false;  // the same for the 16 lanes of a group
+      if constexpr (G == 1) {
+        // One representative: exact early reject. With S the shared-word sum
+        // over q's entries of min(c_r, c_q), T_q the sum of q's stored counts
+        // and S', C' those sums over the entries read so far,
+        // S <= S' + (T_q - C'): once the deficit C' - S' exceeds T_q - wc_q
+        // the pair fails the test (func.cpp:567) whatever the unread entries
+        // hold. Full batches of 128 entries, each followed by a 16-lane sum
+        // of the deficit (the batch count is the same for the whole group).
+        constexpr int NB1 = 8;
+        const int slack = mask ? J.st.ksum[q] - wcq : 0;
+        cut = mask != 0 && slack < 0;
+        int deficit = 0;
+        int64_t b = e0;
+        if (!cut) {
+          for (; b + NB1 * 16 <= e1; b += NB1 * 16) {
+            uint32_t xs[NB1];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) xs[i] = J.st.kp[e + 16 * i];
+            for (int i = 0; i < NB1; ++i) xs[i] = J.st.kp[b + gl + 16 * i];
+            int sg = 0, dg = 0;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (!((mask >> g) & 1u)) continue;
-          const uint16_t* tg = table + g * nw;
-          int sg = 0;
+            for (int i = 0; i < NB1; ++i) {
+              const uint32_t cnt = xs[i] >> 16;
+              const uint32_t m = min((uint32_t)table[xs[i] & 0xFFFFu], cnt);
+              sg += (int)m;
+              dg += (int)(cnt - m);
+            }
+            acc[0] += sg;
+            deficit += dg;
+            int d = deficit;
 #pragma unroll
-          for (int i = 0; i < NB; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
-          acc[g] += sg;
+            for (int off = 8; off > 0; off >>= 1) d += __shfl_xor(d, off, 16);
+            if (d > slack) {
+              cut = true;
+              b += NB1 * 16;  // entries read
+              break;
+            }
+          }
         }
+        e = cut ? e1 : b + gl;
+        if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)((cut ? min(b, e1) : e1) - e0);
+      } else {
+        // the profile loads are latency bound: NB, then 4, in flight per
+        // lane. With 16 tables every lookup runs, whether or not its
+        // representative is still in play (its total is discarded below):
+        // the per-table branches would hold 16 divergent exec masks in
+        // scalar registers.
+        constexpr int NB = 8;
+        for (; e + (NB - 1) * 16 < e1; e += NB * 16) {
+          uint32_t xs[NB];
+#pragma unroll
+          for (int i = 0; i < NB; ++i) xs[i] = J.st.kp[e + 16 * i];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (G < 16 && !((mask >> g) & 1u)) continue;
+            const uint16_t* tg = table + g * kTS;
+            int sg = 0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) sg += min((uint32_t)tg[xs[i] & 0xFFFFu], xs[i] >> 16);
+            acc[g] += sg;
+          }
+        }
+        if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
       }
       for (; e + 48 < e1; e += 64) {
         const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          if (!((mask >> g) & 1u)) continue;
-          const uint16_t* tg = table + g * nw;
+          if (G < 16 && !((mask >> g) & 1u)) continue;
+          const uint16_t* tg = table + g * kTS;
           acc[g] += min((uint32_t)tg[x0 & 0xFFFFu], x0 >> 16) + min((uint32_t)tg[x1 & 0xFFFFu], x1 >> 16) +
                     min((uint32_t)tg[x2 & 0xFFFFu], x2 >> 16) + min((uint32_t)tg[x3 & 0xFFFFu], x3 >> 16);
         }
@@ -284,12 +342,12 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         const uint32_t x = J.st.kp[e];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          if (!((mask >> g) & 1u)) continue;
-          acc[g] += min((uint32_t)table[g * nw + (x & 0xFFFFu)], x >> 16);
+          if (G < 16 && !((mask >> g) & 1u)) continue;
+          acc[g] += min((uint32_t)table[g * kTS + (x & 0xFFFFu)], x >> 16);
         }
       }
-      if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
-      const int wcq = drop ? 0 : J.st.wc[q];
+      // a cut pair fails: its partial sum is dropped (wc_q >= 1)
+      if (cut) acc[0] = 0;
       // per representative g of the group: the 16-lane total, kept by lane g
       int mine = 0;
 #pragma unroll
@@ -1162,7 +1220,7 @@ __device__ inline void align_loop64(const AlignArgs& a) {
 // grid of 8 blocks per CU be resident), 4 up to 10 words per lane, then 3
 // and 2 (measured: the widest tiles cannot fit more without spilling).
 __host__ __device__ constexpr int align_waves_per_eu(int s, int w) {
-  return s < 64 ? 8 : w <= 10 ? 4 : w <= 12 ? 3 : 2;
+  return s < 64 ? 8 : w <= 8 ? 4 : w <= 12 ? 3 : 2;
 }
 
 template <int S, int W>
@@ -1185,6 +1243,8 @@ int align_blocks_per_cu(int shape);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s);
+// Per sequence, the sum of the counts stored in its k-mer profile.
+hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n, int32_t* ksum, hipStream_t s);
 // Cutoffs of n sequences for threshold t and word length k (func.cpp:339-374).
 hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
