@@ -10,6 +10,7 @@
 //   orc_kmap, orc_cutoffs   exhaustive tables vs exact rational arithmetic and
 //                           the float32 quirk counts; k boundaries 0.88/0.94
 //   orc_summin              brute-force multiset intersection (Python Counter)
+//   orc_shared_words_pass   early stop == full scan at wc = sum-1, sum, sum+1
 //   orc_lcs                 textbook O(nm) LCS DP; random/adversarial lengths
 //   orc_decide_fast         == (orc_lcs > bc) on random + near-threshold pairs
 //   orc_sort_order          std::sort over records holding std::string data
@@ -102,6 +103,7 @@ struct Seq {
   int comp[4] = {0, 0, 0, 0};
   std::vector<uint64_t> lo, hi;          // gap-free bit planes, 64 bases/word
   std::vector<std::pair<uint16_t, uint16_t>> words;  // (word, count) distinct
+  int total = 0;             // sum of the stored counts
   int bc = 0, wc = 0;
 };
 
@@ -131,6 +133,7 @@ void prepare(const std::string& data, int k, float t, Seq& s) {
     size_t j = i;
     while (j < w.size() && w[j] == w[i]) ++j;
     s.words.emplace_back(w[i], (uint16_t)(j - i));
+    s.total += (uint16_t)(j - i);
     i = j;
   }
   // deleteGap + compressData func.cpp:375-446 (64 bases per word here).
@@ -243,6 +246,27 @@ bool decide_fast(const Seq& text, const Seq& query, int bc, int64_t* cells) {
   return zeros() >= c;
 }
 
+// kernel_filter func.cpp:552-567: sum over q's distinct words of
+// min(table[w], count) >= wc_q. With `early`, the scan stops as soon as the
+// deficit (sum of count - min) exceeds total - wc: the sum over the words
+// still unread can then not lift the shared count to wc (exact; the decision
+// equals the full scan's, tests/test_oracle.py compares the two).
+bool shared_words_pass(const std::vector<uint16_t>& table, const Seq& s, bool early) {
+  if (!early) {
+    int shared = 0;
+    for (auto& e : s.words) shared += std::min(table[e.first], e.second);
+    return shared >= s.wc;
+  }
+  const int slack = s.total - s.wc;
+  if (slack < 0) return false;
+  int deficit = 0;
+  for (auto& e : s.words) {
+    deficit += e.second - std::min(table[e.first], e.second);
+    if (deficit > slack) return false;
+  }
+  return true;
+}
+
 struct Stats {
   int64_t reps = 0, pairs = 0, rej_comp = 0, rej_word = 0, aligned = 0, accepted = 0;
   double seconds = 0;
@@ -290,9 +314,7 @@ void cluster_seqs(const std::vector<Seq>& seqs, int k, int threads, bool fast, b
         for (int b = 0; b < 4; ++b) sum += std::min(r.comp[b], s.comp[b]);
         if (sum < s.bc) { rc++; continue; }
         // kernel_filter func.cpp:544-571
-        int shared = 0;
-        for (auto& e : s.words) shared += std::min(table[e.first], e.second);
-        if (shared < s.wc) { rw++; continue; }
+        if (!shared_words_pass(table, s, fast)) { rw++; continue; }
       }
       // kernel_dynamic func.cpp:658-675
       al++;
@@ -350,6 +372,22 @@ long long orc_summin(const uint8_t* a, int n, const uint8_t* b, int m, int k) {
   long long s = 0;
   for (auto& e : y.words) s += std::min(table[e.first], e.second);
   return s;
+}
+
+// Shared-word test of one pair (raw code strings; q's cutoff wc given):
+// early = 0 scans every word, 1 stops at the exact deficit bound.
+int orc_shared_words_pass(const uint8_t* a, int n, const uint8_t* b, int m, int k, int wc, int early) {
+  static const char letters[5] = {'A', 'C', 'G', 'T', 'N'};
+  std::string da(n, 'A'), db(m, 'A');
+  for (int i = 0; i < n; ++i) da[i] = letters[a[i] > 4 ? 4 : a[i]];
+  for (int i = 0; i < m; ++i) db[i] = letters[b[i] > 4 ? 4 : b[i]];
+  Seq x, y;
+  prepare(da, k, 0.95f, x);
+  prepare(db, k, 0.95f, y);
+  y.wc = wc;
+  std::vector<uint16_t> table((size_t)1 << (2 * k), 0);
+  for (auto& e : x.words) table[e.first] = e.second;
+  return shared_words_pass(table, y, early != 0) ? 1 : 0;
 }
 
 // Length order: perm[s] = original index of the s-th sorted record, by

@@ -101,18 +101,27 @@ def save(d):
     os.replace(tmp, OUT)
 
 
+BIG_JOB = 20000  # records: such a job gets every thread, one job at a time
+
+
 def run_jobs(specs, have, threads, tmp, label, keep):
-    """Oracle runs of many small jobs, several at once (one thread each);
-    keep(list) stores the entries finished so far."""
+    """Oracle runs of many jobs: the small ones several at once (one thread
+    each), then the large ones one after another on every thread; keep(list)
+    stores the entries finished so far."""
     done = {(e["seed"], e["t"]): e for e in have}
     todo = [s for s in specs if (s["seed"], s["t"]) not in done]
+
+    def finish(e):
+        done[(e["seed"], e["t"])] = e
+        keep([done[(s["seed"], s["t"])] for s in specs if (s["seed"], s["t"]) in done])
+        print(f"{label} job n={e['n']} t={e['t']}: reps {e['reps']}, {e['oracle_s']} s", flush=True)
+
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        futs = {ex.submit(oracle_entry, s, s["t"], 1, tmp): s for s in todo}
+        futs = [ex.submit(oracle_entry, s, s["t"], 1, tmp) for s in todo if s["n"] < BIG_JOB]
         for f in cf.as_completed(futs):
-            e = f.result()
-            done[(e["seed"], e["t"])] = e
-            keep([done[(s["seed"], s["t"])] for s in specs if (s["seed"], s["t"]) in done])
-            print(f"{label} job n={e['n']} t={e['t']}: reps {e['reps']}, {e['oracle_s']} s", flush=True)
+            finish(f.result())
+    for s in sorted((s for s in todo if s["n"] >= BIG_JOB), key=lambda s: s["n"]):
+        finish(oracle_entry(s, s["t"], threads, tmp))
     return [done[(s["seed"], s["t"])] for s in specs]
 
 

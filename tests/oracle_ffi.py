@@ -31,6 +31,8 @@ def orc():
         L.orc_decide_fast.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.orc_summin.restype = C.c_longlong
         L.orc_summin.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int]
+        L.orc_shared_words_pass.restype = C.c_int
+        L.orc_shared_words_pass.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_sort_order.restype = None
         L.orc_sort_order.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_cluster_file.restype = C.c_longlong
@@ -71,6 +73,13 @@ def decide_fast(a, b, bc):
 def summin(a, b, k):
     ca, cb = codes(a), codes(b)
     return orc().orc_summin(ca.tobytes(), len(ca), cb.tobytes(), len(cb), k)
+
+
+def shared_words_pass(a, b, k, wc, early):
+    """The shared-word test of q = b against representative a at cutoff wc
+    (early: with the exact deficit stop)."""
+    ca, cb = codes(a), codes(b)
+    return bool(orc().orc_shared_words_pass(ca.tobytes(), len(ca), cb.tobytes(), len(cb), k, wc, 1 if early else 0))
 
 
 def cutoffs(L, t, k):

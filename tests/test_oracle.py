@@ -128,6 +128,51 @@ def test_summin_lowercase_and_u():
     assert O.summin(b"acgtuACGU", b"ACGTTACGT", 4) == O.summin(b"ACGTTACGT", b"ACGTTACGT", 4)
 
 
+@pytest.mark.parametrize("k", [4, 5, 6, 7, 8])
+def test_shared_words_early_stop_matches_full_scan(k):
+    """The exact deficit stop of the shared-word scan (fast mode) decides like
+    the full scan (func.cpp:552-567) at the cutoffs around the true shared
+    sum S: wc = S (pass) and S + 1 (fail) are the cases a wrong bound would
+    flip; wc above the candidate's total (no scan at all) and far below it
+    are covered too."""
+    rng = np.random.default_rng(100 + k)
+    checked = 0
+    for i in range(60):
+        n = int(rng.integers(1, 1500))
+        a = rand_seq(rng, n, b"ACGTACGTACGTN" if i % 4 == 0 else b"ACGT")
+        kind = i % 3
+        if kind == 0:
+            b = rand_seq(rng, int(rng.integers(1, n + 1)))
+        else:
+            b = bytearray(a[int(rng.integers(0, max(1, n // 4))):])
+            for _ in range(int(len(b) * rng.uniform(0, 0.1))):
+                b[int(rng.integers(0, len(b)))] = int(ACGT[rng.integers(0, 4)])
+            b = bytes(b) or b"A"
+        S = O.summin(a, b, k)
+        total = sum(kmer_counter(b, k).values())
+        for wc in {1, max(1, S - 1), max(1, S), S + 1, S + 5, total, total + 1}:
+            full = O.shared_words_pass(a, b, k, wc, early=False)
+            assert full == (S >= wc)
+            assert O.shared_words_pass(a, b, k, wc, early=True) == full, (k, n, len(b), S, total, wc)
+            checked += 1
+    assert checked > 300
+
+
+def test_fast_mode_clusters_like_reference_mode(tmp_path):
+    """The oracle's fast mode (banded early-exit alignment, deficit stop in
+    the shared-word scan) against its reference mode (full LCS, full scan) on
+    whole clusterings, including a t=0.99 job where the stop fires often."""
+    import gsdr
+
+    for law, seed, n, t in ((2, 41, 1500, 0.99), (1, 42, 1200, 0.95), (2, 43, 1200, 0.90)):
+        fa = str(tmp_path / f"in{seed}.fasta")
+        gsdr.Records.generate(law, seed, n).write_fasta(fa)
+        fast, _, sf = O.cluster_file(fa, t, threads=4, fast=True)
+        ref, _, sr = O.cluster_file(fa, t, threads=4, fast=False)
+        assert np.array_equal(fast, ref), (law, t)
+        assert sf["rej_word"] == sr["rej_word"] and sf["aligned"] == sr["aligned"]
+
+
 # --------------------------------------------------------------------- LCS
 def test_lcs_random_small():
     rng = np.random.default_rng(1)
