@@ -271,6 +271,8 @@ def main():
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
     ap.add_argument("--tune", action="store_true", help="autotune the launch knobs on the workload first")
     ap.add_argument("--tune-trials", type=int, default=12)
+    ap.add_argument("--serial-shapes", action="store_true",
+                    help="run the tile-shape kernels of a step phase one after another (profiling)")
     ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
     ap.add_argument("--jobs-count", type=int, default=512)
     ap.add_argument("--jobs-scale", type=float, default=1.0, help="scale the upper job size")
@@ -326,6 +328,13 @@ def main():
         tu = store.autotune(t, k, max_trials=args.tune_trials)
         tuning = tu.as_dict()
         print(f"# autotune: {tuning}", file=sys.stderr, flush=True)
+    if args.serial_shapes:
+        # the tile-shape kernels of a step phase one after another on one
+        # stream (profiling: kernel durations then do not overlap)
+        tu = eng.tuning()
+        tu.fork_shapes = 0
+        eng.set_tuning(tu)
+        tuning = dict(tuning or {}, fork_shapes=0)
     tune = tuning is not None
     ref = None
     for i in range(args.warmup):
