@@ -28,12 +28,14 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   hipError_t e;
   if (ev && (e = hipEventRecord(ev[0], q)) != hipSuccess) return e;
   if ((e = launch_filter(L.fa_reps, L.kmask, L.reps_filter_grid, q)) != hipSuccess) return e;
+  if (L.pigeon && (e = launch_pigeon(L.pa, L.pigeon_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
   if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
   if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
   if ((e = launch_filter(L.fa_main, L.kmask, L.filter_grid, q)) != hipSuccess) return e;
+  if (L.pigeon && (e = launch_pigeon(L.pa, L.pigeon_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
   if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
@@ -86,6 +88,9 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.ra.jobs_left = jobs_left;
   L.ra.stats = w.stats;
   L.ra.pair_budget = w.item_cap;
+  L.pigeon = T.pigeon;
+  L.pigeon_grid = c->cu_count * 4;
+  L.pa = PigeonArgs{w.jobs, w.items, w.item_cap, w.counters, smax, w.stats};
   return L;
 }
 
@@ -287,6 +292,7 @@ int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_p
       total.ms_filter += s1.ms_filter; total.ms_align += s1.ms_align; total.filter_launches += s1.filter_launches;
       total.align_launches += s1.align_launches; total.filter_bytes += s1.filter_bytes;
       total.cells_decided += s1.cells_decided;
+      total.rej_pigeon += s1.rej_pigeon;
     }
     if (stats) *stats = total;
     return GSDR_OK;

@@ -138,6 +138,8 @@ void stats_from_device(const unsigned long long* d, gsdr_stats* o) {
   o->filter_bytes += (double)d[ST_FILTER_BYTES];
   o->spec_wasted += (double)d[ST_SPEC_WASTE];
   o->cells_decided += (double)d[ST_CELLS_DECIDED];
+  o->rej_pigeon += (double)d[ST_REJ_PIGEON];
+  o->pigeon_checked += (double)d[ST_PIGEON_CHECKED];
 }
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
@@ -231,8 +233,13 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
             "filter launch");
     ++launches;
     ++filter_launches;
-    if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
     const int smax = max_shape_for(s.m[first]);
+    if (T.pigeon) {
+      PigeonArgs pa{w.jobs, w.items, w.item_cap, w.counters, smax, w.stats};
+      HIP_TRY(launch_pigeon(pa, c->cu_count * 4, q), "pigeon launch");
+      ++launches;
+    }
+    if (p.timing) HIP_TRY(hipEventRecord(ev[1], q), "event");
     AlignArgs aa[kNumShapes];
     int grids[kNumShapes];
     for (int sh = 0; sh <= smax; ++sh) {

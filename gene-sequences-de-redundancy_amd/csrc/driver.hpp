@@ -38,6 +38,7 @@ struct Tuning {
   int align_blocks = 4;  // 0: each shape kernel's resident blocks per CU
   int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
   int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
+  int pigeon = 0;        // exact pigeonhole prefilter between the filter and the alignment
 };
 
 struct gsdr_ctx {
@@ -169,6 +170,9 @@ struct StepLaunch {
   int align_grid[kNumShapes] = {};
   int reps_grid[kNumShapes] = {};
   int reps_filter_grid = 1;
+  int pigeon = 0;  // pigeonhole test after each filter launch
+  int pigeon_grid = 1;
+  PigeonArgs pa{};
 };
 StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
                             int32_t* jobs_left, const Tuning& T = Tuning());

@@ -216,6 +216,11 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
   }
 }
 
+hipError_t launch_pigeon(const PigeonArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(pigeon_kernel, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------- profile totals
 // One wave per sequence: T_s = sum of the counts (modulo 2^16, as stored) of
 // its distinct k-mers. The shared-word sum with any representative is at most

@@ -64,7 +64,7 @@ __host__ __device__ inline double band_cells_rows(int n, int m, int c, int R) {
 enum StatSlot {
   ST_PAIRS = 0, ST_REJ_LEN, ST_REJ_COMP, ST_REJ_WORD, ST_ALIGNED, ST_ACCEPT, ST_REJ_ALIGN,
   ST_CELLS_EXEC, ST_CELLS_BAND, ST_CELLS_FULL, ST_FILTER_BYTES, ST_SPEC_WASTE, ST_CELLS_DECIDED,
-  ST_OVERFLOW, ST_NUM = 16
+  ST_OVERFLOW, ST_REJ_PIGEON, ST_PIGEON_CHECKED, ST_NUM = 16
 };
 
 // Store arrays resident in HBM (indexed by sorted sequence index).
@@ -165,6 +165,9 @@ struct AlignItem {
   int32_t text, query, job, slot;
 };
 constexpr int kSwapFlag = 256;
+// Set in AlignItem.slot by the pigeonhole prefilter: the pair cannot reach
+// the cutoff and the alignment kernels skip it.
+constexpr int kPigeonFlag = 512;
 constexpr int kSwapRatio = 4;  // default; a tuned context may choose another (0 = never)
 
 __host__ __device__ inline bool swap_roles(int n_rep, int m_cand, int ratio = kSwapRatio) {

@@ -19,6 +19,15 @@ struct FilterArgs {
   unsigned long long* stats;
 };
 
+struct PigeonArgs {
+  const JobDev* jobs;
+  AlignItem* items;         // the per-shape lists: items + shape * item_cap
+  int64_t item_cap;
+  const int32_t* bin_count; // items per shape
+  int32_t smax;             // shapes 0..smax
+  unsigned long long* stats;
+};
+
 struct AlignArgs {
   const JobDev* jobs;
   const AlignItem* items;   // this shape's list
@@ -424,6 +433,84 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
   }
 }
 
+// ------------------------------------------------------- pigeonhole test
+// Exact prefilter between the filter and the alignment (no counterpart in
+// the reference). For a pair with gap-free lengths n, m and cutoff c = bc+1,
+// a common subsequence of length >= c leaves at most E = n + m - 2c bases of
+// the two sequences unmatched. Cut the shorter sequence x into E+1 disjoint
+// segments: an unmatched base of x lies in one segment, and an unmatched base
+// of the longer y between two matched bases of x touches only the segment
+// holding both, so some segment has no unmatched base at all and occurs
+// verbatim in y. Its matches lie on one diagonal d = pos_y - pos_x, and every
+// match of such a path has d in [-(|x| - c), |y| - c]. So when no segment
+// occurs in y at an in-band diagonal, LCS < c: the pair fails the alignment
+// test (func.cpp:671) and is flagged here instead of aligned. Checked only
+// where it is cheap: segments of at least 8 bases and (E+1)^2 <= 4|x|
+// segment-diagonal positions.
+__device__ __forceinline__ uint2 window32(const uint2* pk, int64_t off, int pos, int len) {
+  // bases pos .. pos+len-1 (len <= 32, inside the sequence) of both planes
+  const int w = pos >> 5, s = pos & 31;
+  const uint2 a = pk[off + w];
+  uint2 b = make_uint2(0u, 0u);
+  if (s + len > 32) b = pk[off + w + 1];
+  const uint32_t lo = s ? (a.x >> s) | (b.x << (32 - s)) : a.x;
+  const uint32_t hi = s ? (a.y >> s) | (b.y << (32 - s)) : a.y;
+  return make_uint2(lo, hi);
+}
+
+__global__ __launch_bounds__(256) void pigeon_kernel(PigeonArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nw = gridDim.x * 4;
+  unsigned long long rej = 0, checked = 0;
+  for (int sh = 0; sh <= a.smax; ++sh) {
+    const int cnt = a.bin_count[sh];
+    AlignItem* list = a.items + (int64_t)sh * a.item_cap;
+    for (int i = gw; i < cnt; i += nw) {
+      const AlignItem it = list[i];
+      const int text = __builtin_amdgcn_readfirstlane(it.text), query = __builtin_amdgcn_readfirstlane(it.query);
+      const int slot = __builtin_amdgcn_readfirstlane(it.slot);
+      const JobDev& J = a.jobs[__builtin_amdgcn_readfirstlane(it.job)];
+      const StoreView& st = J.st;
+      const int cand = (slot & kSwapFlag) ? text : query;
+      const int n = st.m[text], mq = st.m[query], c = st.bc[cand] + 1;
+      if (c > min(n, mq)) continue;  // the alignment rejects it without work
+      const int x = n <= mq ? text : query, y = n <= mq ? query : text;
+      const int mx = min(n, mq), ny = max(n, mq);
+      const int segs = n + mq - 2 * c + 1;  // E + 1
+      const int len = min(32, mx / segs);
+      if (len < 8 || (long long)segs * segs > 4ll * mx) continue;
+      ++checked;
+      const int64_t xo = st.pkoff[x], yo = st.pkoff[y];
+      const int dlo = -(mx - c);
+      const uint32_t mask = len == 32 ? ~0u : (1u << len) - 1u;
+      bool found = false;
+      for (int j = 0; j < segs && !found; ++j) {
+        const int ax = j * len;
+        const uint2 xs = window32(st.pk, xo, ax, len);
+        for (int d0 = 0; d0 < segs && !found; d0 += 64) {
+          const int d = dlo + d0 + lane;
+          const int by = ax + d;
+          bool hit = false;
+          if (d0 + lane < segs && by >= 0 && by + len <= ny) {
+            const uint2 ys = window32(st.pk, yo, by, len);
+            hit = (((xs.x ^ ys.x) | (xs.y ^ ys.y)) & mask) == 0u;
+          }
+          found = __builtin_amdgcn_ballot_w64(hit) != 0;
+        }
+      }
+      if (!found) {
+        if (lane == 0) list[i].slot = slot | kPigeonFlag;
+        ++rej;
+      }
+    }
+  }
+  if (lane == 0 && (checked | rej)) {
+    atomicAdd(&a.stats[ST_PIGEON_CHECKED], checked);
+    if (rej) atomicAdd(&a.stats[ST_REJ_PIGEON], rej);
+  }
+}
+
 // ------------------------------------------------------------- align body
 template <int S>
 __host__ __device__ constexpr uint64_t top_lanes() {
@@ -690,6 +777,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
         continue;
       }
       const AlignItem it = a.items[got];
+      if (it.slot & kPigeonFlag) continue;  // rejected (and counted) by the pigeonhole test
       const JobDev& J = a.jobs[it.job];
       const StoreView& st = J.st;
       const int islot = it.slot & (kSwapFlag - 1);
@@ -1177,6 +1265,7 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     // lane keeps them (and the control flow that depends on them) scalar
     const AlignItem it = a.items[got];
     const int text = uniform(it.text), query = uniform(it.query), jobi = uniform(it.job), rslot = uniform(it.slot);
+    if (rslot & kPigeonFlag) continue;  // rejected (and counted) by the pigeonhole test
     const JobDev& J = a.jobs[jobi];
     const StoreView& st = J.st;
     const int islot = rslot & (kSwapFlag - 1);
@@ -1245,6 +1334,7 @@ hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budget, hipStream_t s);
 // Per sequence, the sum of the counts stored in its k-mer profile.
 hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n, int32_t* ksum, hipStream_t s);
+hipError_t launch_pigeon(const PigeonArgs& a, int grid, hipStream_t s);
 // Cutoffs of n sequences for threshold t and word length k (func.cpp:339-374).
 hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,

@@ -59,7 +59,9 @@ class Stats(C.Structure):
         ("filter_bytes", C.c_double),
         ("spec_wasted", C.c_double),
         ("cells_decided", C.c_double),
-        ("reserved", C.c_double * 3),
+        ("rej_pigeon", C.c_double),
+        ("pigeon_checked", C.c_double),
+        ("reserved", C.c_double * 1),
     ]
 
     def as_dict(self):
@@ -76,13 +78,13 @@ class Tuning(C.Structure):
         ("trials", C.c_int32),
         ("fork_shapes", C.c_int32),
         ("list_factor", C.c_int32),
-        ("reserved2", C.c_int32),
+        ("pigeon", C.c_int32),
         ("ms_default", C.c_double),
         ("ms_tuned", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved2"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Options(C.Structure):

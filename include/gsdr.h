@@ -81,7 +81,10 @@ typedef struct gsdr_stats {
   double filter_bytes;  /* algorithmic bytes read+written by the filter kernels */
   double spec_wasted;   /* batched: pair evaluations discarded by speculation */
   double cells_decided; /* band cells of the rows each exact decision needed (32-row steps) */
-  double reserved[3];
+  double rej_pigeon;    /* filtered pairs rejected by the exact pigeonhole test (tuning.pigeon = 1);
+                           aligned = accepted + rej_align + spec_wasted + rej_pigeon */
+  double pigeon_checked; /* filtered pairs the pigeonhole test examined */
+  double reserved[1];
 } gsdr_stats;
 
 /* Launch knobs that change speed, never results. */
@@ -95,7 +98,7 @@ typedef struct gsdr_tuning {
   int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
   int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
                              sequence (default 64); it also bounds k_spec * candidates */
-  int32_t reserved2;
+  int32_t pigeon;         /* 1 = exact pigeonhole prefilter before the alignment (sound; default 0) */
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
 } gsdr_tuning;

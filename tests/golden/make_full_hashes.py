@@ -145,15 +145,15 @@ def main():
             d["single"][key] = e
             save(d)
             print(f"{key}: reps {e['reps']}, oracle {e['oracle_s']} s", flush=True)
-        if not want or "many" in want:
-            specs = W.many_jobs(MANY_COUNT)
-            d["many"] = run_jobs(specs, [] if args.redo else d.get("many", []), args.threads, tmp, "many",
-                                 lambda v: (d.__setitem__("many", v), save(d)))
-            save(d)
         if not want or "stream" in want:
             specs = W.stream_jobs(STREAM_COUNT)
             d["stream"] = run_jobs(specs, [] if args.redo else d.get("stream", []), args.threads, tmp, "stream",
                                    lambda v: (d.__setitem__("stream", v), save(d)))
+            save(d)
+        if not want or "many" in want:
+            specs = W.many_jobs(MANY_COUNT)
+            d["many"] = run_jobs(specs, [] if args.redo else d.get("many", []), args.threads, tmp, "many",
+                                 lambda v: (d.__setitem__("many", v), save(d)))
             save(d)
 
 

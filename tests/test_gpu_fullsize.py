@@ -97,8 +97,11 @@ def test_many_jobs_in_one_call_match_oracle_digests(engine, tmp_path):
     0.80..0.99) clustered together in one batched call; every job's
     cluster[] and result.fasta against the oracle's."""
     jobs = FIX["many"]
-    if len(jobs) != 64:
-        pytest.skip("the oracle digests of the 64-job sample are not in the fixture yet")
+    # the fixture holds the jobs whose oracle run has finished (the sample
+    # is 64 jobs; the slowest oracle runs may be missing)
+    if len(jobs) < 60:
+        pytest.skip(f"only {len(jobs)} of the 64 jobs have oracle digests")
+    print(f"\n# many jobs: {len(jobs)} with oracle digests", flush=True)
     recs, stores = [], []
     try:
         for e in jobs:
@@ -128,8 +131,9 @@ def test_stream_jobs_match_oracle_digests(engine):
     admitted while earlier ones run; every job's cluster[] against the
     oracle's."""
     jobs = FIX["stream"]
-    if len(jobs) != 64:
-        pytest.skip("the oracle digests of the 64-job stream sample are not in the fixture yet")
+    if len(jobs) < 60:
+        pytest.skip(f"only {len(jobs)} of the 64 stream jobs have oracle digests")
+    print(f"\n# stream jobs: {len(jobs)} with oracle digests", flush=True)
     specs = [{"id": i, "n": e["n"], "t": e["t"], "seed": e["seed"], "law": 2, "median": e["median"]}
              for i, e in enumerate(jobs)]
     recs = [W.generate(s, sort=True) for s in specs]
@@ -138,7 +142,7 @@ def test_stream_jobs_match_oracle_digests(engine):
         got = {r["job"]: r["cluster"] for r in out}
         bad = [i for i, e in enumerate(jobs) if sha_cluster(got[i]) != e["cluster"]]
         assert not bad, f"jobs {bad} differ from the oracle"
-        assert summary["jobs"] == 64 and summary["max_concurrent"] > 1
+        assert summary["jobs"] == len(jobs) and summary["max_concurrent"] > 1
     finally:
         for r in recs:
             r.free()
