@@ -47,9 +47,11 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
       (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.state, (size_t)1)) || (st = dev_alloc(A, F, &s.raw_len_d, (size_t)n)) ||
-      (st = dev_alloc(A, F, &s.ksum, (size_t)n)))
+      (st = dev_alloc(A, F, &s.ksum, (size_t)n)) || (st = dev_alloc(A, F, &s.rec, (size_t)n)))
     return st;
+  if (s.kp_total >= (int64_t)1 << 32) return fail(GSDR_E_LIMIT, "more than 2^32 k-mer profile entries in one store");
   HIP_TRY(launch_profile_totals(s.kp, s.kpoff, (int)n, s.ksum, c->upload), "profile totals");
+  HIP_TRY(launch_pack_records(s.m_d, s.comp, s.kpoff, s.ksum, (int)n, s.rec, c->upload), "pack records");
   HIP_TRY(hipStreamSynchronize(c->upload), "profile totals");
   HIP_TRY(hipMemcpy(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice),
           "upload lengths");
@@ -104,7 +106,7 @@ void workspace_release(gsdr_ctx* c) {
 int store_set_threshold(gsdr_store& s, float t) {
   if (s.t_loaded == t) return GSDR_OK;
   hipStream_t q = s.ctx->upload;
-  HIP_TRY(launch_cutoffs(s.raw_len_d, s.n, t, s.k, s.bc, s.wc, q), "cutoff launch");
+  HIP_TRY(launch_cutoffs(s.raw_len_d, s.n, t, s.k, s.bc, s.wc, s.rec, q), "cutoff launch");
   HIP_TRY(hipStreamSynchronize(q), "cutoff sync");
   s.t_loaded = t;
   return GSDR_OK;

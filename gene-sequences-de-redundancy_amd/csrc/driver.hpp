@@ -83,6 +83,8 @@ struct gsdr_store {
   int32_t* wc = nullptr;
   int32_t* raw_len_d = nullptr;     // raw lengths (cutoffs are computed from them on the device)
   int32_t* ksum = nullptr;          // per sequence: sum of its stored k-mer counts
+  gsdr::SeqRec* rec = nullptr;      // per sequence: the filter's packed record
+  int64_t kp_total = 0;             // profile entries of all sequences
   float t_loaded = -1.0f;
   // per-job round state (allocated with the store)
   int32_t* cluster = nullptr;
@@ -96,6 +98,7 @@ struct gsdr_store {
     gsdr::StoreView v;
     v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
     v.ksum = ksum;
+    v.rec = rec;
     return v;
   }
   gsdr::JobDev job(const gsdr_params& p) const {

@@ -251,21 +251,55 @@ hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n
 // factor forced to 1 (func.cpp:353-366). Each product and difference is
 // rounded on its own (no contraction), so the values equal the host's.
 __global__ __launch_bounds__(256) void cutoff_kernel(const int32_t* __restrict__ raw_len, int n, float t, int k,
-                                                     int32_t* __restrict__ bc, int32_t* __restrict__ wc) {
+                                                     int32_t* __restrict__ bc, int32_t* __restrict__ wc,
+                                                     SeqRec* __restrict__ rec) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int L = raw_len[i];
   const float fl = (float)L;
-  bc[i] = (int)ceilf(__fmul_rn(fl, t));
+  const int b = (int)ceilf(__fmul_rn(fl, t));
   const float ce = ceilf(__fmul_rn(fl, __fsub_rn(1.0f, t)));
   const int cut = (int)__fmul_rn(ce, (float)k);
   const int req = L - k + 1 - cut;
-  wc[i] = req < 1 ? 1 : req;
+  const int w = req < 1 ? 1 : req;
+  bc[i] = b;
+  wc[i] = w;
+  if (rec) {
+    rec[i].bc = b;
+    rec[i].wc = w;
+  }
 }
 
-hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s) {
+hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, SeqRec* rec,
+                          hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(cutoff_kernel, dim3((n + 255) / 256), dim3(256), 0, s, raw_len, n, t, k, bc, wc);
+  hipLaunchKernelGGL(cutoff_kernel, dim3((n + 255) / 256), dim3(256), 0, s, raw_len, n, t, k, bc, wc, rec);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void pack_records_kernel(const int32_t* __restrict__ m, const uint4* __restrict__ comp,
+                                                           const int64_t* __restrict__ kpoff,
+                                                           const int32_t* __restrict__ ksum, int n,
+                                                           SeqRec* __restrict__ rec) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint4 cp = comp[i];
+  SeqRec r;
+  r.m = m[i];
+  r.bc = 0;  // set with the threshold (cutoff_kernel)
+  r.wc = 0;
+  r.ksum = ksum[i];
+  r.comp_ac = cp.x | (cp.y << 16);
+  r.comp_gt = cp.z | (cp.w << 16);
+  r.kp0 = (uint32_t)kpoff[i];
+  r.kn = (uint32_t)(kpoff[i + 1] - kpoff[i]);
+  rec[i] = r;
+}
+
+hipError_t launch_pack_records(const int32_t* m, const uint4* comp, const int64_t* kpoff, const int32_t* ksum, int n,
+                               SeqRec* rec, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_records_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m, comp, kpoff, ksum, n, rec);
   return hipGetLastError();
 }
 

@@ -67,6 +67,16 @@ enum StatSlot {
   ST_OVERFLOW, ST_REJ_PIGEON, ST_PIGEON_CHECKED, ST_NUM = 16
 };
 
+// The filter's view of one sequence, one 32-byte record (a candidate costs
+// one cache line instead of one per array): gap-free length, cutoffs, the
+// sum of its stored k-mer counts, composition and its profile's place.
+struct __align__(16) SeqRec {
+  int32_t m, bc, wc, ksum;
+  uint32_t comp_ac, comp_gt;  // counts of A | C << 16, G | T << 16 (each <= 32,767)
+  uint32_t kp0, kn;           // first profile entry and entries
+};
+static_assert(sizeof(SeqRec) == 32, "one 32-byte record per sequence");
+
 // Store arrays resident in HBM (indexed by sorted sequence index).
 struct StoreView {
   const uint2* pk;         // (lo plane, hi plane) per 32 gap-free bases
@@ -78,6 +88,7 @@ struct StoreView {
   const int32_t* bc;       // ceil(L*t) (func.cpp:368)
   const int32_t* wc;       // shared-word cutoff (func.cpp:353-366)
   const int32_t* ksum;     // sum of the stored k-mer counts (the profile's total)
+  const SeqRec* rec;       // the filter's per-sequence records
 };
 
 // Speculative representatives per step are bounded by the accept-mask width.

@@ -130,6 +130,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
   __shared__ int s_pre[kMaxJobsPerLaunch + 1];
   __shared__ int s_part[4];
   __shared__ int s_slot[G], s_rep[G], s_n[G];
+  __shared__ uint32_t s_kp0[G], s_kn[G];
   __shared__ uint4 s_comp[G];
   const int tid = threadIdx.x;
   const int nj = a.njobs;
@@ -198,13 +199,15 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         const int r = cand[slot];
         s_slot[tid] = slot;
         s_rep[tid] = r;
-        s_n[tid] = J.st.m[r];
-        s_comp[tid] = J.st.comp[r];
+        const SeqRec rr = J.st.rec[r];
+        s_n[tid] = rr.m;
+        s_comp[tid] = make_uint4(rr.comp_ac & 0xFFFFu, rr.comp_ac >> 16, rr.comp_gt & 0xFFFFu, rr.comp_gt >> 16);
+        s_kp0[tid] = rr.kp0;
+        s_kn[tid] = rr.kn;
       }
       __syncthreads();
       for (int g = 0; g < gcount; ++g) {
-        const int r = s_rep[g];
-        const int64_t k0 = J.st.kpoff[r], k1 = J.st.kpoff[r + 1];
+        const int64_t k0 = s_kp0[g], k1 = k0 + s_kn[g];
         uint16_t* tg = table + g * kTS;
         for (int64_t e = k0 + tid; e < k1; e += 256) {
           const uint32_t x = J.st.kp[e];
@@ -220,8 +223,9 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     // group have their index and record loads issued together.
     constexpr int NS = filter_subs(G);
     const bool drop = (J.flags & kJobDropFilters) != 0;
-    int qv[NS], mqv[NS], bcv[NS];
-    uint4 cqv[NS];
+    // each candidate's record: (m, bc, wc, ksum) and (composition, profile)
+    int qv[NS];
+    uint4 rav[NS], rbv[NS];
 #pragma unroll
     for (int sub = 0; sub < NS; ++sub) {
       const int ps = p0 + chunk * filter_chunk(G) + sub * kFilterChunk + grp;
@@ -229,20 +233,20 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     }
 #pragma unroll
     for (int sub = 0; sub < NS; ++sub) {
-      mqv[sub] = J.st.m[qv[sub]];
-      bcv[sub] = J.st.bc[qv[sub]];
-      cqv[sub] = J.st.comp[qv[sub]];
+      const uint4* rp = reinterpret_cast<const uint4*>(J.st.rec + qv[sub]);
+      rav[sub] = rp[0];
+      rbv[sub] = rp[1];
     }
 #pragma unroll
     for (int sub = 0; sub < NS; ++sub) {
       const int pos = p0 + chunk * filter_chunk(G) + sub * kFilterChunk + grp;
       const bool inpos = pos < p1;
       const int q = qv[sub];
-      const int mq = mqv[sub];
-      const int bcq = bcv[sub];
+      const int mq = (int)rav[sub].x;
+      const int bcq = (int)rav[sub].y;
       const int c = bcq + 1;
-      const uint4 cq = cqv[sub];
-      if (gl == 0 && inpos) bytes += 28;  // id, m, bc, comp
+      const uint4 cq = make_uint4(rbv[sub].x & 0xFFFFu, rbv[sub].x >> 16, rbv[sub].y & 0xFFFFu, rbv[sub].y >> 16);
+      if (gl == 0 && inpos) bytes += 36;  // id, record
       // length and composition per representative of the group
       uint32_t mask = 0;
 #pragma unroll
@@ -263,7 +267,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       // shared words: q's profile read once, one table lookup per survivor so
       // far (none when no representative is left; drop mode reads no profile
       // and passes every pair)
-      const int64_t e0 = J.st.kpoff[q], e1 = (drop || mask == 0) ? e0 : J.st.kpoff[q + 1];
+      const int64_t e0 = rbv[sub].z, e1 = (drop || mask == 0) ? e0 : e0 + rbv[sub].w;
       int acc[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) acc[g] = 0;
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       // With 16 tables every lookup runs, whether or not its representative
       // is still in play (its total is discarded below): the per-table
       // branches would hold 16 divergent exec masks in scalar registers.
-      const int wcq = drop ? 0 : J.st.wc[q];
+      const int wcq = drop ? 0 : (int)rav[sub].z;
       int64_t e = e0 + gl;
       bool cut = false;  // the same for the 16 lanes of a group
       if constexpr (G == 1) {
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         // hold. Full batches of 128 entries, each followed by a 16-lane sum
         // of the deficit (the batch count is the same for the whole group).
         constexpr int NB1 = 8;
-        const int slack = mask ? J.st.ksum[q] - wcq : 0;
+        const int slack = mask ? (int)rav[sub].w - wcq : 0;
         cut = mask != 0 && slack < 0;
         int deficit = 0;
         int64_t b = e0;
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
           }
         }
         e = cut ? e1 : b + gl;
-        if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)((cut ? min(b, e1) : e1) - e0);
+        if (gl == 0 && mask != 0) bytes += 4ull * (unsigned long long)((cut ? min(b, e1) : e1) - e0);
       } else {
         // the profile loads are latency bound: NB, then 4, in flight per
         // lane. With 16 tables every lookup runs, whether or not its
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
             acc[g] += sg;
           }
         }
-        if (gl == 0 && mask != 0) bytes += 20 + 4ull * (unsigned long long)(e1 - e0);  // kpoff pair, wc, profile
+        if (gl == 0 && mask != 0) bytes += 4ull * (unsigned long long)(e1 - e0);  // the profile
       }
       for (; e + 48 < e1; e += 64) {
         const uint32_t x0 = J.st.kp[e], x1 = J.st.kp[e + 16], x2 = J.st.kp[e + 32], x3 = J.st.kp[e + 48];
@@ -1336,7 +1340,12 @@ hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budge
 hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n, int32_t* ksum, hipStream_t s);
 hipError_t launch_pigeon(const PigeonArgs& a, int grid, hipStream_t s);
 // Cutoffs of n sequences for threshold t and word length k (func.cpp:339-374).
-hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, hipStream_t s);
+// (also into the records' bc and wc when rec is not null)
+hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, SeqRec* rec,
+                          hipStream_t s);
+// Fills the records' length, profile total, composition and profile place.
+hipError_t launch_pack_records(const int32_t* m, const uint4* comp, const int64_t* kpoff, const int32_t* ksum, int n,
+                               SeqRec* rec, hipStream_t s);
 hipError_t launch_init_jobs(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, int k_max,
                             int32_t* jobs_left, hipStream_t s);
 

@@ -255,6 +255,7 @@ int store_build_device(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s,
       st = fail(GSDR_E_LIMIT, "sequence longer than 32767 gap-free bases");
       break;
     }
+    s.kp_total = kpoff[(size_t)n];
     if ((st = alloc(s, (void**)&s.pk, sizeof(uint2) * (size_t)pkoff[(size_t)n])) ||
         (st = alloc(s, (void**)&s.kp, sizeof(uint32_t) * (size_t)std::max<int64_t>(kpoff[(size_t)n], 1))))
       break;
