@@ -142,17 +142,11 @@ def bench_jobs(args):
 
     import gsdr
 
-    rng = np.random.default_rng(1003)
     nj = args.jobs_count
-    lo, hi = np.log(500), np.log(50000 * args.jobs_scale)
-    specs = [{"n": int(round(np.exp(rng.uniform(lo, hi)))), "median": float(rng.choice([150, 400, 1500, 5000])),
-              "t": float(rng.choice([0.80, 0.85, 0.90, 0.95, 0.99])), "seed": int(1003 * 1000003 + j)}
-             for j in range(nj)]
+    specs = W.many_jobs(nj, size_range=(500, int(50000 * args.jobs_scale)))
 
     def gen(s):
-        r = gsdr.Records.generate(2, s["seed"], s["n"], s["median"])
-        r.sort()
-        return r
+        return W.generate(s, sort=True)
 
     t0 = time.perf_counter()
     recs = parallel_map(gen, specs)
@@ -179,10 +173,12 @@ def bench_jobs(args):
     # each job alone must give the same clusters (spot check)
     same = all(np.array_equal(stores[j].cluster(ts[j], stores[j].k, mode=gsdr.MODE_BATCHED)[0], outs[j])
                for j in range(0, nj, max(1, nj // 8)))
-    t0 = time.perf_counter()
-    for j in range(nj):
-        stores[j].cluster(ts[j], stores[j].k, mode=gsdr.MODE_PLAIN)
-    plain_s = time.perf_counter() - t0
+    plain_s = None
+    if not args.no_compare:
+        t0 = time.perf_counter()
+        for j in range(nj):
+            stores[j].cluster(ts[j], stores[j].k, mode=gsdr.MODE_PLAIN)
+        plain_s = time.perf_counter() - t0
     ms = 1e3 * sum(times) / len(times)
     out = {"metric": "clustered Mbp/s, many independent jobs in shared launches", "value": round(bases / 1e6 / (ms / 1e3), 3),
            "unit": "Mbp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 3),
@@ -191,7 +187,8 @@ def bench_jobs(args):
            "config": {"workload": f"many jobs: {nj} jobs, sizes log-uniform [500, {int(50000 * args.jobs_scale)}], "
                                   "medians {150,400,1500,5000}, t in {.80,.85,.90,.95,.99}",
                       "records": int(sum(len(r) for r in recs)), "bases": int(bases)},
-           "plain_sequential_s": round(plain_s, 3), "batched_over_plain": round(plain_s / (ms / 1e3), 3),
+           "plain_sequential_s": round(plain_s, 3) if plain_s else None,
+           "batched_over_plain": round(plain_s / (ms / 1e3), 3) if plain_s else None,
            "alone_equals_mix_sample": bool(same), "reps": int(st.reps)}
     for s in stores:
         s.free()
@@ -201,7 +198,8 @@ def bench_jobs(args):
 
 def bench_serve(args):
     """Poisson arrivals of clustering jobs at several offered loads,
-    admitted by HBM footprint into the continuous-batching server."""
+    admitted by HBM footprint into the continuous-batching server. Jobs are
+    generated by loader threads ahead of their arrival."""
     import numpy as np
 
     import gsdr
@@ -212,18 +210,21 @@ def bench_serve(args):
     results = []
     for ri, rate in enumerate(rates):
         count = max(4, int(round(rate * args.duration)))
-        specs = serve.make_jobs(1005 + ri, count, size_range=(1000, int(200000 * args.jobs_scale)))
-        t0 = time.perf_counter()
-        recs = parallel_map(serve.generate_job, specs)
-        print(f"# rate {rate}: {count} jobs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-        out, summary = serve.run_stream(eng, specs, recs, rate, hbm_budget=int(args.budget_gb * 2**30),
-                                        max_slots=args.slots, seed=ri, loaders=3)
+        specs = W.stream_jobs(count, seed=1005 + ri, size_range=(1000, int(200000 * args.jobs_scale)))
+        print(f"# rate {rate}: {count} jobs over {args.duration:.0f} s", file=sys.stderr, flush=True)
+        out, summary = serve.run_stream(eng, specs, None, rate, hbm_budget=int(args.budget_gb * 2**30),
+                                        max_slots=args.slots, seed=ri, loaders=args.loaders,
+                                        make=lambda sp: W.generate(sp, sort=True), progress_s=30.0,
+                                        drain_s=args.drain)
+        # the first jobs again, each alone: the same clusters as in the mix
         check = []
-        for r, s, rec in list(zip(out, specs, recs))[:3]:
-            cl, _ = gsdr.cluster_records(eng, rec, s["t"], mode=gsdr.MODE_BATCHED)
+        for r in out[:3]:
+            rec = W.generate(specs[r["job"]], sort=True)
+            cl, _ = gsdr.cluster_records(eng, rec, specs[r["job"]]["t"], mode=gsdr.MODE_BATCHED)
             check.append(bool(np.array_equal(cl, r["cluster"])))
+            rec.free()
         summary["alone_equals_mix_first3"] = all(check)
-        summary["offered_mbp_per_s"] = sum(r.bases for r in recs) / 1e6 / max(1e-9, count / rate)
+        summary["offered_mbp_per_s"] = summary["offered_mbp"] / max(1e-9, count / rate)
         print(f"# rate {rate}: {json.dumps(summary)}", file=sys.stderr, flush=True)
         results.append(summary)
     eng.close()
@@ -233,7 +234,7 @@ def bench_serve(args):
                       "higher_is_better": True, "data": "synthetic (seeded generator of the stated law)",
                       "config": {"workload": f"Poisson job stream: sizes log-uniform [1000, {int(200000 * args.jobs_scale)}], "
                                              f"t in {{.80,.90,.95,.99}}, {args.duration:.0f} s of arrivals per rate",
-                                 "budget_gb": args.budget_gb, "slots": args.slots},
+                                 "budget_gb": args.budget_gb, "slots": args.slots, "drain_s": args.drain},
                       "rates": results}), flush=True)
 
 
@@ -282,6 +283,8 @@ def main():
     ap.add_argument("--duration", type=float, default=20.0, help="seconds of arrivals per rate")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--loaders", type=int, default=8, help="serving: host threads generating and submitting jobs")
+    ap.add_argument("--drain", type=float, default=300.0, help="serving: seconds after the last arrival to finish")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

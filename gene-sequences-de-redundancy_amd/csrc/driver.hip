@@ -52,9 +52,12 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
   if (s.kp_total >= (int64_t)1 << 32) return fail(GSDR_E_LIMIT, "more than 2^32 k-mer profile entries in one store");
   HIP_TRY(launch_profile_totals(s.kp, s.kpoff, (int)n, s.ksum, c->upload), "profile totals");
   HIP_TRY(launch_pack_records(s.m_d, s.comp, s.kpoff, s.ksum, (int)n, s.rec, c->upload), "pack records");
-  HIP_TRY(hipStreamSynchronize(c->upload), "profile totals");
-  HIP_TRY(hipMemcpy(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice),
+  // every upload goes through the upload stream: a null-stream copy here could
+  // meet a graph capture that the serving loop runs on another thread
+  HIP_TRY(hipMemcpyAsync(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                         c->upload),
           "upload lengths");
+  HIP_TRY(hipStreamSynchronize(c->upload), "profile totals");
   return GSDR_OK;
 }
 

@@ -45,6 +45,9 @@ struct gsdr_server {
   int64_t seqs = 0;  // sequences of the waiting and running jobs
   int64_t next_id = 1;
   std::mutex mu;
+  // held by submissions while they build a store on the device and by the
+  // step while it captures its graph: loads never run inside a capture
+  std::mutex dev_mu;
   std::map<int64_t, std::unique_ptr<ServeJob>> jobs;
   std::vector<ServeJob*> waiting;  // loaded, not yet in the device table
   std::vector<ServeJob*> running;  // in the device table
@@ -139,15 +142,20 @@ int gsdr_serve_submit(gsdr_server* srv, const gsdr_records* sorted, const gsdr_p
     job->reserved = need;
     job->id = srv->next_id++;
   }
-  // load outside the lock (host preprocessing + upload on the upload stream)
+  // load outside the bookkeeping lock, on the upload stream, never during a
+  // graph capture of the step
   job->store = std::make_unique<gsdr_store>();
   job->store->ctx = srv->c;
-  int st = store_build(srv->c, r, k, *job->store);
-  int kk = 0;
-  if (!st) st = check_params(*job->store, *params, &kk);
-  if (!st) st = store_set_threshold(*job->store, t);
+  int st;
+  {
+    std::lock_guard<std::mutex> dk(srv->dev_mu);
+    st = store_build(srv->c, r, k, *job->store);
+    int kk = 0;
+    if (!st) st = check_params(*job->store, *params, &kk);
+    if (!st) st = store_set_threshold(*job->store, t);
+    if (st) store_release(*job->store);
+  }
   if (st) {
-    store_release(*job->store);
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->used -= job->reserved;
     srv->seqs -= n;
@@ -205,6 +213,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
     // reallocated the workspace the graph's kernels point into
     if (!srv->exec || srv->graph_jobs != srv->running || srv->graph_ws != c->ws.generation) {
       const StepLaunch L = make_step_launch(c, c->ws, nj, smax, kmask, nullptr);
+      std::lock_guard<std::mutex> dk(srv->dev_mu);
       if ((e = capture_steps(srv, L, q)) != hipSuccess) return hip_fail(e, "serve capture");
     }
     if ((e = hipGraphLaunch(srv->exec, q)) != hipSuccess) return hip_fail(e, "serve steps");
