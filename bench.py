@@ -273,6 +273,7 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune the launch knobs on the workload first")
     ap.add_argument("--tune-trials", type=int, default=12)
     ap.add_argument("--pigeon", action="store_true", help="enable the exact pigeonhole prefilter")
+    ap.add_argument("--no-screen", action="store_true", help="turn the word-signature screen off")
     ap.add_argument("--serial-shapes", action="store_true",
                     help="run the tile-shape kernels of a step phase one after another (profiling)")
     ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
@@ -338,6 +339,12 @@ def main():
         tu.pigeon = 1
         eng.set_tuning(tu)
         tuning = dict(tuning or {}, pigeon=1)
+    if args.no_screen:
+        # the phase-B filter without the word-signature screen in front
+        tu = eng.tuning()
+        tu.screen = 0
+        eng.set_tuning(tu)
+        tuning = dict(tuning or {}, screen=0)
     if args.serial_shapes:
         # the tile-shape kernels of a step phase one after another on one
         # stream (profiling: kernel durations then do not overlap)
@@ -405,7 +412,8 @@ def main():
         "stage": {"pairs": st.pairs, "rej_length": st.rej_length, "rej_comp": st.rej_comp,
                   "rej_word": st.rej_word, "aligned": st.aligned, "accepted": st.accepted,
                   "rej_align": st.rej_align, "align_skipped": int(st.spec_wasted),
-                  "pigeon_checked": int(st.pigeon_checked), "rej_pigeon": int(st.rej_pigeon)},
+                  "pigeon_checked": int(st.pigeon_checked), "rej_pigeon": int(st.rej_pigeon),
+                  "screened": int(st.screened)},
         "cells": {"band": st.cells_band, "decided": st.cells_decided, "exec": st.cells_exec,
                   "full": st.cells_full},
         "kernel_ms": {"align": round(st.ms_align, 3), "filter": round(st.ms_filter, 3),

@@ -358,7 +358,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
   if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || t->filter_blocks < 1 ||
       t->filter_blocks > 64 || t->align_blocks < 0 || t->align_blocks > 64 ||
       (t->fork_shapes != 0 && t->fork_shapes != 1) || t->list_factor < 1 || t->list_factor > 256 ||
-      (t->pigeon != 0 && t->pigeon != 1))
+      (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1))
     return fail(GSDR_E_INVALID, "tuning out of range");
   Tuning T;
   T.swap_ratio = t->swap_ratio;
@@ -368,6 +368,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
   T.fork_shapes = t->fork_shapes;
   T.list_factor = t->list_factor;
   T.pigeon = t->pigeon;
+  T.screen = t->screen;
   c->tuned = T;
   c->has_tuning = true;
   return GSDR_OK;
@@ -384,6 +385,7 @@ int gsdr_get_tuning(const gsdr_ctx* c, gsdr_tuning* out) {
   out->fork_shapes = T.fork_shapes;
   out->list_factor = T.list_factor;
   out->pigeon = T.pigeon;
+  out->screen = T.screen;
   return GSDR_OK;
 }
 

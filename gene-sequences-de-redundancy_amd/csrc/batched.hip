@@ -34,6 +34,7 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
   if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
+  if (L.screen && (e = launch_screen(L.sa, L.screen_grid, q)) != hipSuccess) return e;
   if ((e = launch_filter(L.fa_main, L.kmask, L.filter_grid, q)) != hipSuccess) return e;
   if (L.pigeon && (e = launch_pigeon(L.pa, L.pigeon_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
@@ -67,8 +68,14 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.fa_main.item_cap = w.item_cap;
   L.fa_main.bin_count = w.counters;
   L.fa_main.stats = w.stats;
+  // the screen only runs for jobs whose stores carry signatures (k >= 6)
+  L.screen = T.screen && (kmask >> kScreenMinK) != 0;
+  L.fa_main.screen = L.screen;
+  L.screen_grid = c->cu_count * 8;
+  L.sa = ScreenArgs{w.jobs, njobs, w.stats};
   L.fa_reps = L.fa_main;
   L.fa_reps.phase = PHASE_REPS;
+  L.fa_reps.screen = 0;
   for (int sh = 0; sh < kNumShapes; ++sh) {
     AlignArgs& aa = L.aa[sh];
     aa.jobs = w.jobs;
@@ -88,6 +95,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.ra.jobs_left = jobs_left;
   L.ra.stats = w.stats;
   L.ra.pair_budget = w.item_cap;
+  L.ra.grid = c->cu_count * 2;
   L.pigeon = T.pigeon;
   L.pigeon_grid = c->cu_count * 4;
   L.pa = PigeonArgs{w.jobs, w.items, w.item_cap, w.counters, smax, w.stats};
@@ -153,7 +161,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   int32_t left = 0;
   for (int j = 0; j < njobs; ++j) left += jobs[j].n > 0;
   HIP_TRY(hipMemcpyAsync(w.jobs, jobs, sizeof(JobDev) * njobs, hipMemcpyHostToDevice, q), "job table");
-  HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.stats, 0, kStatWords * sizeof(unsigned long long), q), "memset");
   HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs, njobs, k_max, w.jobs_left, q), "init");
@@ -167,6 +175,10 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   int64_t launches = 2, steps_launched = 0;
   // a job needs at most n steps; every step fixes at least one representative
   const int64_t step_limit = max_n + 2;
+  // kernels per step: two filters, resolve_reps and the three resolve
+  // kernels, the shape kernels of both phases, the clamp with several jobs,
+  // the screen and the pigeonhole tests when on
+  const int64_t per_step = 6 + 2 * (L.smax + 1) + (njobs > 1 ? 1 : 0) + L.screen + 2 * L.pigeon;
   if (timing) {
     // timed form: the same step launched directly with events around the
     // filter and the alignment kernels
@@ -174,7 +186,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
     while (true) {
       HIP_TRY(launch_step(c, L, q, ev), "step launch");
-      launches += 3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0);
+      launches += per_step;
       ++steps_launched;
       HIP_TRY(hipMemcpyAsync(c->h_state, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
       HIP_TRY(hipStreamSynchronize(q), "step sync");
@@ -199,7 +211,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       HIP_TRY(hipMemcpyAsync(h_left, w.jobs_left, sizeof(int32_t), hipMemcpyDeviceToHost, q), "poll");
       HIP_TRY(hipStreamSynchronize(q), "step sync");
       steps_launched += B;
-      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0));
+      launches += (int64_t)B * per_step;
       if (*h_left == 0) break;
       if (steps_launched > step_limit + B) return fail(GSDR_E_DEVICE, "steps did not converge");
     }
@@ -228,7 +240,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
       if (e == hipSuccess) e = hipStreamSynchronize(q);
       if (e != hipSuccess) { rc = hip_fail(e, "graph replay"); break; }
       steps_launched += B;
-      launches += (int64_t)B * (3 + 2 * (L.smax + 1) + 1 + (njobs > 1 ? 1 : 0));
+      launches += (int64_t)B * per_step;
       if (*h_left == 0) break;
       if (steps_launched > step_limit + B) { rc = fail(GSDR_E_DEVICE, "steps did not converge"); break; }
     }
@@ -239,14 +251,15 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   for (int j = 0; j < njobs; ++j)
     HIP_TRY(hipMemcpyAsync(out[j], jobs[j].cluster, (size_t)jobs[j].n * sizeof(int32_t), hipMemcpyDeviceToHost, q),
             "cluster copy");
-  unsigned long long hst[ST_NUM];
-  HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
+  unsigned long long hall[kStatWords], hst[ST_NUM];
+  HIP_TRY(hipMemcpyAsync(hall, w.stats, sizeof(hall), hipMemcpyDeviceToHost, q), "stats copy");
   JobState hs1[1];
   std::vector<JobState> hs_many((size_t)(njobs > 1 ? njobs : 0));
   JobState* hs = njobs > 1 ? hs_many.data() : hs1;
   for (int j = 0; j < njobs; ++j)
     HIP_TRY(hipMemcpyAsync(&hs[j], jobs[j].state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");
+  stats_fold(hall, hst);
   if (hst[ST_OVERFLOW]) return fail(GSDR_E_LIMIT, "alignment list overflow");
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {
@@ -265,7 +278,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     stats->ms_total = ms;
     stats->ms_filter = ms_filter;
     stats->ms_align = ms_align;
-    stats->filter_launches = 2 * steps_launched;
+    stats->filter_launches = (2 + L.screen) * steps_launched;
     stats->align_launches = steps_launched * 2 * (L.smax + 1);
   }
   return GSDR_OK;
@@ -293,6 +306,8 @@ int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_p
       total.align_launches += s1.align_launches; total.filter_bytes += s1.filter_bytes;
       total.cells_decided += s1.cells_decided;
       total.rej_pigeon += s1.rej_pigeon;
+      total.pigeon_checked += s1.pigeon_checked;
+      total.screened += s1.screened;
     }
     if (stats) *stats = total;
     return GSDR_OK;

@@ -47,11 +47,19 @@ int store_build(gsdr_ctx* c, const Records& sorted, int k, gsdr_store& s) {
       (st = dev_alloc(A, F, &s.cluster, (size_t)n)) || (st = dev_alloc(A, F, &s.cand[0], (size_t)n)) ||
       (st = dev_alloc(A, F, &s.cand[1], (size_t)n)) || (st = dev_alloc(A, F, &s.acc, (size_t)n)) ||
       (st = dev_alloc(A, F, &s.state, (size_t)1)) || (st = dev_alloc(A, F, &s.raw_len_d, (size_t)n)) ||
-      (st = dev_alloc(A, F, &s.ksum, (size_t)n)) || (st = dev_alloc(A, F, &s.rec, (size_t)n)))
+      (st = dev_alloc(A, F, &s.ksum, (size_t)n)) || (st = dev_alloc(A, F, &s.rec, (size_t)n)) ||
+      (st = dev_alloc(A, F, &s.tcnt, (size_t)(n / kResolveTile + 2))))
+    return st;
+  // signature screen arrays for long words (kSigWords * 4 + 8 bytes per
+  // sequence, one mask per 64 phase-B positions)
+  if (k >= kScreenMinK &&
+      ((st = dev_alloc(A, F, &s.sig, (size_t)n * kSigWords)) || (st = dev_alloc(A, F, &s.scr, (size_t)n)) ||
+       (st = dev_alloc(A, F, &s.chm, (size_t)(n / 64 + 1)))))
     return st;
   if (s.kp_total >= (int64_t)1 << 32) return fail(GSDR_E_LIMIT, "more than 2^32 k-mer profile entries in one store");
   HIP_TRY(launch_profile_totals(s.kp, s.kpoff, (int)n, s.ksum, c->upload), "profile totals");
   HIP_TRY(launch_pack_records(s.m_d, s.comp, s.kpoff, s.ksum, (int)n, s.rec, c->upload), "pack records");
+  if (s.sig) HIP_TRY(launch_signatures(s.kp, s.kpoff, (int)n, k, s.sig, c->upload), "signatures");
   // every upload goes through the upload stream: a null-stream copy here could
   // meet a graph capture that the serving loop runs on another thread
   HIP_TRY(hipMemcpyAsync(s.raw_len_d, s.raw_len.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
@@ -79,7 +87,7 @@ int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs) {
   const int64_t cap = std::max<int64_t>(item_cap, 1024);
   if ((st = dev_alloc(A, F, &w.items, (size_t)cap * kNumShapes)) ||
       (st = dev_alloc(A, F, &w.counters, (size_t)2 * kNumShapes)) ||
-      (st = dev_alloc(A, F, &w.stats, (size_t)ST_NUM)) ||
+      (st = dev_alloc(A, F, &w.stats, (size_t)kStatWords)) ||
       (st = dev_alloc(A, F, &w.jobs, (size_t)std::max(njobs, 1))) || (st = dev_alloc(A, F, &w.jobs_left, (size_t)1))) {
     for (void* p : A) (void)hipFree(p);
     w = Workspace();
@@ -129,6 +137,14 @@ int check_params(const gsdr_store& s, const gsdr_params& p, int* k_out) {
   return GSDR_OK;
 }
 
+void stats_fold(const unsigned long long* all, unsigned long long* out) {
+  for (int i = 0; i < ST_NUM; ++i) {
+    unsigned long long v = 0;
+    for (int r = 0; r < kStatStripes; ++r) v += all[r * kStatStride + i];
+    out[i] = v;
+  }
+}
+
 void stats_from_device(const unsigned long long* d, gsdr_stats* o) {
   o->pairs += (int64_t)d[ST_PAIRS];
   o->rej_length += (int64_t)d[ST_REJ_LEN];
@@ -145,6 +161,7 @@ void stats_from_device(const unsigned long long* d, gsdr_stats* o) {
   o->cells_decided += (double)d[ST_CELLS_DECIDED];
   o->rej_pigeon += (double)d[ST_REJ_PIGEON];
   o->pigeon_checked += (double)d[ST_PIGEON_CHECKED];
+  o->screened += (double)d[ST_SCREENED];
 }
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats) {
@@ -208,7 +225,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   const int n = s.n;
   const JobDev jd = s.job(p);
   HIP_TRY(hipMemcpyAsync(w.jobs, &jd, sizeof(JobDev), hipMemcpyHostToDevice, q), "job table");
-  HIP_TRY(hipMemsetAsync(w.stats, 0, ST_NUM * sizeof(unsigned long long), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.stats, 0, kStatWords * sizeof(unsigned long long), q), "memset");
   HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
   HIP_TRY(launch_init_jobs(w.jobs, &jd, 1, 1, nullptr, q), "init");
   int64_t launches = 1;
@@ -234,6 +251,13 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     fa.item_cap = w.item_cap;
     fa.bin_count = w.counters;
     fa.stats = w.stats;
+    fa.screen = T.screen && s.sig != nullptr;
+    if (fa.screen) {
+      const ScreenArgs sa{w.jobs, 1, w.stats};
+      HIP_TRY(launch_screen(sa, std::max(1, std::min((nc + 63) / 64, c->cu_count * 8)), q), "screen launch");
+      ++launches;
+      ++filter_launches;
+    }
     HIP_TRY(launch_filter(fa, 1u << k, std::max(1, std::min(units, c->cu_count * T.filter_blocks)), q),
             "filter launch");
     ++launches;
@@ -271,8 +295,9 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     ra.jobs_left = nullptr;
     ra.stats = w.stats;
     ra.pair_budget = w.item_cap;
+    ra.grid = std::max(1, std::min((nc + kResolveTile - 1) / kResolveTile, c->cu_count * 2));
     HIP_TRY(launch_resolve(ra, q), "resolve launch");
-    ++launches;
+    launches += 3;
     HIP_TRY(hipMemcpyAsync(c->h_state, s.state, sizeof(JobState), hipMemcpyDeviceToHost, q), "state copy");
     HIP_TRY(hipStreamSynchronize(q), "round sync");
     if (p.timing) {
@@ -289,9 +314,10 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     done = h.done != 0;
   }
   HIP_TRY(hipMemcpyAsync(out, s.cluster, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, q), "cluster copy");
-  unsigned long long hst[ST_NUM];
-  HIP_TRY(hipMemcpyAsync(hst, w.stats, sizeof(hst), hipMemcpyDeviceToHost, q), "stats copy");
+  unsigned long long hall[kStatWords], hst[ST_NUM];
+  HIP_TRY(hipMemcpyAsync(hall, w.stats, sizeof(hall), hipMemcpyDeviceToHost, q), "stats copy");
   HIP_TRY(hipStreamSynchronize(q), "final sync");
+  stats_fold(hall, hst);
   if (hst[ST_OVERFLOW]) return fail(GSDR_E_LIMIT, "alignment list overflow");
   if (p.timing)
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -376,7 +402,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
       (st = dev_alloc(A, F, &d_m, (size_t)nslots)) || (st = dev_alloc(A, F, &d_bc, (size_t)nslots)) ||
       (st = dev_alloc(A, F, &items, (size_t)npairs)) || (st = dev_alloc(A, F, &res, (size_t)npairs)) ||
       (st = dev_alloc(A, F, &ctr, (size_t)2 * kNumShapes)) || (st = dev_alloc(A, F, &d_job, (size_t)1)) ||
-      (st = dev_alloc(A, F, &d_stats, (size_t)ST_NUM))) {
+      (st = dev_alloc(A, F, &d_stats, (size_t)kStatWords))) {
     release();
     return st;
   }
@@ -404,7 +430,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
       (e = hipMemcpyAsync(items, all.data(), all.size() * sizeof(AlignItem), hipMemcpyHostToDevice, q)) ||
       (e = hipMemcpyAsync(ctr, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, q)) ||
       (e = hipMemcpyAsync(d_job, &jd, sizeof(jd), hipMemcpyHostToDevice, q)) ||
-      (e = hipMemsetAsync(d_stats, 0, ST_NUM * 8, q)))
+      (e = hipMemsetAsync(d_stats, 0, kStatWords * 8, q)))
     return bail(e, "align_pairs upload");
   for (int sh = 0; sh < kNumShapes; ++sh) {
     if (counts[sh] == 0) continue;
@@ -422,10 +448,11 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return bail(e, "align launch");
   }
   std::vector<int32_t> got(all.size());
-  unsigned long long hst[ST_NUM];
+  unsigned long long hall[kStatWords], hst[ST_NUM];
   if ((e = hipMemcpyAsync(got.data(), res, got.size() * 4, hipMemcpyDeviceToHost, q)) ||
-      (e = hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, q)) || (e = hipStreamSynchronize(q)))
+      (e = hipMemcpyAsync(hall, d_stats, sizeof(hall), hipMemcpyDeviceToHost, q)) || (e = hipStreamSynchronize(q)))
     return bail(e, "align_pairs download");
+  stats_fold(hall, hst);
   for (int sh = 0; sh < kNumShapes; ++sh)
     for (size_t j = 0; j < bin_pair[sh].size(); ++j) accept_out[bin_pair[sh][j]] = got[first[sh] + j];
   if (cells_exec_out) *cells_exec_out = (double)hst[ST_CELLS_EXEC];

@@ -39,6 +39,7 @@ struct Tuning {
   int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
   int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
   int pigeon = 0;        // exact pigeonhole prefilter between the filter and the alignment
+  int screen = 1;        // word-signature screen before the phase-B filter (k >= 6)
 };
 
 struct gsdr_ctx {
@@ -84,6 +85,10 @@ struct gsdr_store {
   int32_t* raw_len_d = nullptr;     // raw lengths (cutoffs are computed from them on the device)
   int32_t* ksum = nullptr;          // per sequence: sum of its stored k-mer counts
   gsdr::SeqRec* rec = nullptr;      // per sequence: the filter's packed record
+  uint32_t* sig = nullptr;          // word-presence signatures (k >= 6), else null
+  unsigned long long* scr = nullptr;  // screen survivors per sequence (with sig)
+  unsigned long long* chm = nullptr;  // screen survivors per phase-B chunk (with sig)
+  int32_t* tcnt = nullptr;          // resolve: survivors per tile
   int64_t kp_total = 0;             // profile entries of all sequences
   float t_loaded = -1.0f;
   // per-job round state (allocated with the store)
@@ -99,6 +104,7 @@ struct gsdr_store {
     v.pk = pk; v.pkoff = pkoff; v.m = m_d; v.comp = comp; v.kp = kp; v.kpoff = kpoff; v.bc = bc; v.wc = wc;
     v.ksum = ksum;
     v.rec = rec;
+    v.sig = sig;
     return v;
   }
   gsdr::JobDev job(const gsdr_params& p) const {
@@ -110,6 +116,9 @@ struct gsdr_store {
     j.cand[1] = cand[1];
     j.acc = acc;
     j.state = state;
+    j.scr = scr;
+    j.chm = chm;
+    j.tcnt = tcnt;
     j.n = n;
     j.k = k;
     return j;
@@ -154,6 +163,8 @@ int cluster_jobs(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsdr_p
                  int32_t* const* out, gsdr_stats* stats);
 int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq, const int32_t* pairs,
                 const int32_t* bc, int npairs, int shape, int32_t* accept_out, double* cells_exec_out);
+// Sums the kStatStripes copies of the device counters into ST_NUM values.
+void stats_fold(const unsigned long long* all, unsigned long long* out);
 void stats_from_device(const unsigned long long* d, gsdr_stats* out);
 }  // namespace gsdr
 
@@ -176,6 +187,9 @@ struct StepLaunch {
   int pigeon = 0;  // pigeonhole test after each filter launch
   int pigeon_grid = 1;
   PigeonArgs pa{};
+  int screen = 0;  // signature screen before the phase-B filter
+  int screen_grid = 1;
+  ScreenArgs sa{};
 };
 StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
                             int32_t* jobs_left, const Tuning& T = Tuning());

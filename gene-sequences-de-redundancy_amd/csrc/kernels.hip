@@ -33,56 +33,93 @@ __global__ __launch_bounds__(64) void resolve_reps_kernel(ResolveArgs a) {
   const int tid = threadIdx.x;
   if (blockIdx.x == 0)
     for (int i = tid; i < a.ncounters; i += 64) a.counters[i] = 0;
-  if (tid != 0) return;
   const JobDev& J = a.jobs[blockIdx.x];
   JobState* S = J.state;
   if (S->done) return;
   const int K = S->k_spec;
   const int32_t* cand = J.cand[S->cur];
+  // lane i holds slot i's representative and accept mask (bits of earlier
+  // slots only: phase A pairs each slot with the later ones)
+  int r = 0;
+  unsigned long long acc = 0;
+  if (tid < K) {
+    r = cand[tid];
+    acc = J.acc[r];
+  }
+  // the chain over the slots, the same in every lane
   unsigned long long tm = 0;
-  int nonrep = 0;
   for (int i = 0; i < K; ++i) {
-    const int r = cand[i];
-    const unsigned long long msk = J.acc[r] & tm;
-    if (msk == 0) {
-      tm |= 1ull << i;
-      J.cluster[r] = r;
-    } else {
-      J.cluster[r] = cand[__builtin_ctzll(msk)];
-      ++nonrep;
-    }
+    const unsigned long long ai = __shfl(acc, i);
+    if ((ai & tm) == 0) tm |= 1ull << i;
+  }
+  if (tid < K) {
+    const unsigned long long msk = acc & tm;
+    J.cluster[r] = msk == 0 ? r : cand[__builtin_ctzll(msk)];
     J.acc[r] = 0;
   }
-  S->tmask = tm;
-  S->kt = __popcll(tm);
-  S->pad[0] = nonrep;
+  if (tid == 0) {
+    S->tmask = tm;
+    S->kt = __popcll(tm);
+    S->nonrep = K - __popcll(tm);
+  }
 }
 
-__global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
-  __shared__ int s_part[16];
+// Phase-B positions K..nc-1 of every running job in tiles of kResolveTile:
+// the prefix of tile counts over the jobs, in LDS; returns the total.
+__device__ int resolve_tiles(const ResolveArgs& a, int* s_pre, int* s_part) {
+  const int tid = threadIdx.x, nj = a.njobs;
+  int carry = 0;
+  for (int base = 0; base < nj; base += 256) {
+    const int j = base + tid;
+    int u = 0;
+    if (j < nj) {
+      const JobState* S = a.jobs[j].state;
+      const int rest = S->done ? 0 : S->nc - S->k_spec;
+      u = rest > 0 ? (rest + kResolveTile - 1) / kResolveTile : 0;
+    }
+    int tot;
+    const int ex = block_exclusive_scan<256>(u, &tot, s_part);
+    if (j < nj) s_pre[j] = carry + ex;
+    carry += tot;
+  }
+  if (tid == 0) s_pre[nj] = carry;
+  __syncthreads();
+  return s_pre[nj];
+}
+
+__device__ __forceinline__ int tile_job(const int* s_pre, int nj, int u) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_pre[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Resolve, part 1 (after the step's alignments): every candidate that a
+// true representative accepted joins the first such one; accept masks are
+// cleared; each tile's survivors are counted.
+__global__ __launch_bounds__(256) void resolve_mark_kernel(ResolveArgs a) {
+  __shared__ int s_pre[kMaxJobsPerLaunch + 1];
+  __shared__ int s_part[4];
   const int tid = threadIdx.x;
   if (blockIdx.x == 0)
-    for (int i = tid; i < a.ncounters; i += 1024) a.counters[i] = 0;
-  const JobDev& J = a.jobs[blockIdx.x];
-  JobState* S = J.state;
-  if (S->done) return;
-  const int K = S->k_spec, nc = S->nc, cur = S->cur;
-  const int32_t* cand = J.cand[cur];
-  int32_t* out = J.cand[cur ^ 1];
-  const unsigned long long tm = S->tmask;
-  if (tid < K && ((tm >> tid) & 1ull)) J.cluster[cand[tid]] = cand[tid];
-  int absorbed = 0;
-  int written = 0;
-  for (int base = K; base < nc; base += 4096) {
-    int qv[4], keep[4], cnt = 0;
+    for (int i = tid; i < a.ncounters; i += 256) a.counters[i] = 0;
+  const int total = resolve_tiles(a, s_pre, s_part);
+  for (int u = blockIdx.x; u < total; u += gridDim.x) {
+    const int job = tile_job(s_pre, a.njobs, u);
+    const JobDev& J = a.jobs[job];
+    JobState* S = J.state;
+    const int K = S->k_spec, nc = S->nc;
+    const int tile = u - s_pre[job];
+    const int32_t* cand = J.cand[S->cur];
+    const unsigned long long tm = S->tmask;
+    int keep = 0, absorbed = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int p = base + tid * 4 + k;
-      qv[k] = -1;
-      keep[k] = 0;
+    for (int i = 0; i < kResolveTile / 256; ++i) {
+      const int p = K + tile * kResolveTile + i * 256 + tid;
       if (p < nc) {
         const int q = cand[p];
-        qv[k] = q;
         const unsigned long long acc = J.acc[q];
         unsigned long long msk = 0;
         if (acc) {
@@ -93,59 +130,120 @@ __global__ __launch_bounds__(1024) void resolve_kernel(ResolveArgs a) {
           J.cluster[q] = cand[__builtin_ctzll(msk)];
           ++absorbed;
         } else {
-          keep[k] = 1;
+          ++keep;
         }
       }
-      cnt += keep[k];
     }
-    int tot;
-    int at = written + block_exclusive_scan<1024>(cnt, &tot, s_part);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (keep[k]) out[at++] = qv[k];
-    written += tot;
+    keep = block_sum<256>(keep, s_part);
+    absorbed = block_sum<256>(absorbed, s_part);
+    if (tid == 0) {
+      J.tcnt[tile] = keep;
+      if (absorbed) atomicAdd(&S->absorbed, absorbed);
+    }
   }
-  absorbed = block_sum<1024>(absorbed, s_part);
-  if (tid == 0) {
-    const int true_reps = __popcll(tm);
-    const int nc2 = written;
-    S->reps += true_reps;
-    S->steps += 1;
-    S->cur = cur ^ 1;
-    S->nc = nc2;
-    int k2 = 1;
-    if (S->k_max > 1) {
-      // f = absorption per true representative and candidate. A step of K
-      // slots evaluates about K*f/2 pairs per candidate whose candidate an
-      // earlier slot already took (fewer where the align fetch skips them),
-      // and phase A costs K^2/2 pairs: aim at K*f near 1/2.
-      const double denom = (double)max(true_reps, 1) * (double)max(nc - K, 1);
-      const double f = (double)absorbed / denom;
-      const int up = min(2 * K, S->k_max), down = max(1, K / 2);
-      if (f <= 0.0) {
-        k2 = up;
-      } else {
-        const double want = 0.5 / f;
-        k2 = want >= up ? up : (want <= down ? down : (int)want);
+}
+
+// Resolve, part 2: the survivors of each tile, in order, after those of the
+// tiles before it (updateRemain, func.cpp:477-487); the job's last tile
+// leaves the total.
+__global__ __launch_bounds__(256) void resolve_scatter_kernel(ResolveArgs a) {
+  __shared__ int s_pre[kMaxJobsPerLaunch + 1];
+  __shared__ int s_part[4];
+  const int tid = threadIdx.x;
+  const int total = resolve_tiles(a, s_pre, s_part);
+  constexpr int PER = kResolveTile / 256;
+  for (int u = blockIdx.x; u < total; u += gridDim.x) {
+    const int job = tile_job(s_pre, a.njobs, u);
+    const JobDev& J = a.jobs[job];
+    JobState* S = J.state;
+    const int K = S->k_spec, nc = S->nc, cur = S->cur;
+    const int tile = u - s_pre[job];
+    const int tiles = (nc - K + kResolveTile - 1) / kResolveTile;
+    int before = 0;
+    for (int t = tid; t < tile; t += 256) before += J.tcnt[t];
+    before = block_sum<256>(before, s_part);
+    const int32_t* cand = J.cand[cur];
+    int32_t* out = J.cand[cur ^ 1];
+    const int p0 = K + tile * kResolveTile + tid * PER;
+    int qv[PER], cnt = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int p = p0 + i;
+      qv[i] = -1;
+      if (p < nc) {
+        const int q = cand[p];
+        if (J.cluster[q] < 0) {
+          qv[i] = q;
+          ++cnt;
+        }
       }
     }
-    // every survivor of a step may land in one shape list: k_spec * nc of
-    // all jobs must stay within its capacity (this job alone here; the
-    // clamp that follows the step bounds the sum over the jobs)
-    const long long cap = a.pair_budget / (long long)max(nc2, 1);
-    if ((long long)k2 > cap) k2 = (int)cap;
-    if (k2 > nc2) k2 = nc2;
-    if (k2 < 1) k2 = 1;
-    S->k_spec = k2;
-    // with one slot, phase A has nothing to decide: slot 0 is the
-    // representative (updateRepresentative, func.cpp:466-476)
-    S->tmask = 1ull;
-    S->kt = 1;
-    S->first = nc2 > 0 ? out[0] : -1;
-    if (nc2 == 0) {
-      S->done = 1;
-      if (a.jobs_left) atomicSub(a.jobs_left, 1);
+    int tot;
+    int at = before + block_exclusive_scan<256>(cnt, &tot, s_part);
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (qv[i] >= 0) out[at++] = qv[i];
+    if (tid == 0 && tile == tiles - 1) S->kept = before + tot;
+  }
+}
+
+// Resolve, part 3: the round state of every job for the next step.
+__global__ __launch_bounds__(256) void resolve_state_kernel(ResolveArgs a) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.njobs) return;
+  const JobDev& J = a.jobs[j];
+  JobState* S = J.state;
+  if (S->done) return;
+  const int K = S->k_spec, nc = S->nc, cur = S->cur;
+  const int32_t* cand = J.cand[cur];
+  const unsigned long long tm = S->tmask;
+  // the true representatives are their own clusters (with one slot this is
+  // the round's representative, updateRepresentative, func.cpp:466-476)
+  for (unsigned long long m = tm; m; m &= m - 1) {
+    const int r = cand[__builtin_ctzll(m)];
+    J.cluster[r] = r;
+  }
+  const int true_reps = __popcll(tm);
+  const int absorbed = S->absorbed;
+  const int nc2 = nc > K ? S->kept : 0;
+  S->absorbed = 0;
+  S->kept = 0;
+  S->reps += true_reps;
+  S->steps += 1;
+  S->cur = cur ^ 1;
+  S->nc = nc2;
+  int k2 = 1;
+  if (S->k_max > 1) {
+    // f = absorption per true representative and candidate. A step of K
+    // slots evaluates about K*f/2 pairs per candidate whose candidate an
+    // earlier slot already took (fewer where the align fetch skips them),
+    // and phase A costs K^2/2 pairs: aim at K*f near 1/2.
+    const double denom = (double)max(true_reps, 1) * (double)max(nc - K, 1);
+    const double f = (double)absorbed / denom;
+    const int up = min(2 * K, S->k_max), down = max(1, K / 2);
+    if (f <= 0.0) {
+      k2 = up;
+    } else {
+      const double want = 0.5 / f;
+      k2 = want >= up ? up : (want <= down ? down : (int)want);
     }
+  }
+  // every survivor of a step may land in one shape list: k_spec * nc of
+  // all jobs must stay within its capacity (this job alone here; the
+  // clamp that follows the step bounds the sum over the jobs)
+  const long long cap = a.pair_budget / (long long)max(nc2, 1);
+  if ((long long)k2 > cap) k2 = (int)cap;
+  if (k2 > nc2) k2 = nc2;
+  if (k2 < 1) k2 = 1;
+  S->k_spec = k2;
+  // with one slot, phase A has nothing to decide: slot 0 is the
+  // representative (updateRepresentative, func.cpp:466-476)
+  S->tmask = 1ull;
+  S->kt = 1;
+  S->first = nc2 > 0 ? J.cand[cur ^ 1][0] : -1;
+  if (nc2 == 0) {
+    S->done = 1;
+    if (a.jobs_left) atomicSub(a.jobs_left, 1);
   }
 }
 
@@ -207,7 +305,7 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
     S->done = n > 0 ? 0 : 1;
     S->kt = 1;
     S->tmask = 1ull;
-    S->pad[0] = S->pad[1] = S->pad[2] = 0;
+    S->nonrep = S->absorbed = S->kept = 0;
     S->pad2 = 0ull;
     S->steps = 0;
     S->reps = 0;
@@ -218,6 +316,17 @@ __global__ __launch_bounds__(256) void init_jobs_kernel(const JobDev* __restrict
 
 hipError_t launch_pigeon(const PigeonArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL(pigeon_kernel, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen(const ScreenArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(screen_kernel, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_signatures(const uint32_t* kp, const int64_t* kpoff, int n, int k, uint32_t* sig, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(signature_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, kp, kpoff, n, k, sig);
   return hipGetLastError();
 }
 
@@ -304,20 +413,29 @@ hipError_t launch_pack_records(const int32_t* m, const uint4* comp, const int64_
 }
 
 // -------------------------------------------------------------- launchers
-template <int G>
-static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+template <int G, bool SCR>
+static hipError_t launch_filter_gs(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
   constexpr int ts = filter_table_stride(G);
   if (G > 1 && (1 << (2 * max_k)) > ts) return hipErrorInvalidValue;  // the tables would overlap
   const size_t lds = G > 1 ? (size_t)G * ts * sizeof(uint16_t) : (sizeof(uint16_t) << (2 * max_k));
   static bool attr_set = false;
   if (lds > 32768 && !attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void*)filter_kernel<G, SCR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)(sizeof(uint16_t) << 16));
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(filter_kernel<G>, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL((filter_kernel<G, SCR>), dim3(grid), dim3(256), lds, s, a);
   return hipGetLastError();
+}
+
+// Phase B of the long-word groups has a form for the signature screen.
+template <int G>
+static hipError_t launch_filter_g(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
+  if constexpr (G <= filter_group(kScreenMinK)) {
+    if (a.screen && a.phase == PHASE_MAIN) return launch_filter_gs<G, true>(a, max_k, grid, s);
+  }
+  return launch_filter_gs<G, false>(a, max_k, grid, s);
 }
 
 static hipError_t launch_filter_k(const FilterArgs& a, int max_k, int grid, hipStream_t s) {
@@ -415,7 +533,10 @@ hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(resolve_kernel, dim3(a.njobs), dim3(1024), 0, s, a);
+  const int grid = a.grid > 0 ? a.grid : 1;
+  hipLaunchKernelGGL(resolve_mark_kernel, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(resolve_scatter_kernel, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(resolve_state_kernel, dim3((a.njobs + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

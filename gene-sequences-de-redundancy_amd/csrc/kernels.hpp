@@ -64,8 +64,14 @@ __host__ __device__ inline double band_cells_rows(int n, int m, int c, int R) {
 enum StatSlot {
   ST_PAIRS = 0, ST_REJ_LEN, ST_REJ_COMP, ST_REJ_WORD, ST_ALIGNED, ST_ACCEPT, ST_REJ_ALIGN,
   ST_CELLS_EXEC, ST_CELLS_BAND, ST_CELLS_FULL, ST_FILTER_BYTES, ST_SPEC_WASTE, ST_CELLS_DECIDED,
-  ST_OVERFLOW, ST_REJ_PIGEON, ST_PIGEON_CHECKED, ST_NUM = 16
+  ST_OVERFLOW, ST_REJ_PIGEON, ST_PIGEON_CHECKED, ST_SCREENED, ST_NUM = 17
 };
+// The counters are striped: kStatStripes copies kStatStride words apart,
+// each wave adding to the copy its block and wave pick, and the host sums
+// the copies (thousands of waves adding to one word serialise on it at the
+// end of every launch). Overflow is only ever added to copy 0.
+constexpr int kStatStripes = 64, kStatStride = 32, kStatWords = kStatStripes * kStatStride;
+static_assert(ST_NUM <= kStatStride, "one stripe holds every counter");
 
 // The filter's view of one sequence, one 32-byte record (a candidate costs
 // one cache line instead of one per array): gap-free length, cutoffs, the
@@ -89,10 +95,27 @@ struct StoreView {
   const int32_t* wc;       // shared-word cutoff (func.cpp:353-366)
   const int32_t* ksum;     // sum of the stored k-mer counts (the profile's total)
   const SeqRec* rec;       // the filter's per-sequence records
+  const uint32_t* sig;     // word-presence signatures (kSigWords per sequence), or null
 };
+
+// Word-presence signature: bit sig_bucket(w) of kSigWords * 32 for every
+// distinct k-mer w of a sequence with a nonzero stored count. Short words map
+// one to one; longer ones through a multiplicative hash (Knuth's 2^32/phi).
+// Each distinct word of q absent from r adds at least 1 to the shared-word
+// deficit of the pair, and a bucket set in q's signature but not in r's holds
+// at least one such word, so popcount(sig_q & ~sig_r) bounds the deficit from
+// below. The bound only fires for long words, where the cutoff leaves little
+// slack (t >= 0.94): shorter words fill the signatures of random sequences.
+constexpr int kSigWords = 64;
+constexpr int kScreenMinK = 7;
+__host__ __device__ inline uint32_t sig_bucket(uint32_t w, int k) {
+  return k <= 5 ? w : (w * 0x9E3779B1u) >> 21;
+}
 
 // Speculative representatives per step are bounded by the accept-mask width.
 constexpr int kMaxSpec = 64;
+// Candidate positions per resolve tile (256 threads, 8 each).
+constexpr int kResolveTile = 2048;
 // Candidates per filter work unit (one per 16-lane group of a 256-thread block).
 constexpr int kFilterChunk = 16;
 // With one table per block (long words) a unit takes four candidates per
@@ -113,7 +136,9 @@ struct JobState {
   int32_t k_max;    // upper bound for k_spec (1 = plain form)
   int32_t first;    // cand[cur][0], the next representative (-1 when done)
   int32_t kt;       // true representatives among the k_spec slots (after phase A)
-  int32_t pad[3];
+  int32_t nonrep;   // speculative slots that an earlier slot absorbed (phase A)
+  int32_t absorbed; // candidates absorbed in phase B (summed over the resolve tiles)
+  int32_t kept;     // candidates left after the step (from the last resolve tile)
   unsigned long long tmask;  // slots that are true representatives (after phase A)
   unsigned long long pad2;
 };
@@ -137,7 +162,7 @@ __host__ __device__ inline int phase_units(const JobState& s, int phase, int gro
 
 // Representatives whose shared-word tables a filter block holds at once:
 // as many as fit 32 KiB of LDS (u16 counts over 4^k words), at most 16.
-__host__ __device__ inline int filter_group(int k) {
+__host__ __device__ constexpr int filter_group(int k) {
   const int bytes = 2 << (2 * k);
   int g = 32768 / bytes;
   if (g >= 16) return 16;
@@ -154,6 +179,12 @@ struct JobDev {
   int32_t* cand[2];          // candidate lists (ping-pong)
   unsigned long long* acc;   // n accept masks: bit i = P(rep slot i, q)
   JobState* state;
+  // signature screen (jobs whose store has signatures): per sequence, the
+  // step's slots whose representative passed the screen against it; per
+  // phase-B chunk of filter_chunk positions, the OR of those masks
+  unsigned long long* scr;
+  unsigned long long* chm;
+  int32_t* tcnt;             // resolve: survivors per tile of kResolveTile positions
   int32_t n;
   int32_t k;                 // word length of the stored profiles
   int32_t flags;             // kJobDropFilters

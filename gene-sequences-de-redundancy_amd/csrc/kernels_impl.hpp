@@ -17,6 +17,13 @@ struct FilterArgs {
   int64_t item_cap;
   int32_t* bin_count;       // kNumShapes counters
   unsigned long long* stats;
+  int32_t screen;           // phase B: skip what the signature screen rejected
+};
+
+struct ScreenArgs {
+  const JobDev* jobs;
+  int32_t njobs;
+  unsigned long long* stats;
 };
 
 struct PigeonArgs {
@@ -46,6 +53,7 @@ struct ResolveArgs {
   int32_t* jobs_left;       // jobs not done (device)
   unsigned long long* stats;
   int64_t pair_budget;      // alignment list capacity per shape (bounds k_spec * nc over the jobs)
+  int32_t grid;             // workgroups of the resolve tile kernels
 };
 
 // ----------------------------------------------------------- block helpers
@@ -92,6 +100,13 @@ __device__ inline int block_exclusive_scan(int v, int* total, int* part) {
   return before + incl - v;
 }
 
+// This wave's copy of the striped counters, picked by the compute unit the
+// wave runs on and its place in the block (read when the counters are
+// flushed, so nothing stays live through the kernel for it).
+__device__ __forceinline__ unsigned long long* stat_row(unsigned long long* st) {
+  return st + (((__smid() << 2) + (threadIdx.x >> 6)) % kStatStripes) * kStatStride;
+}
+
 // Kernel bodies are compiled only in the translation unit that defines
 // GSDR_DEVICE_TU (kernels.hip); the other files see the launchers.
 #ifdef GSDR_DEVICE_TU
@@ -121,9 +136,11 @@ __host__ __device__ constexpr int filter_waves_per_eu(int g) { return g >= 4 ? 4
 // one table of 4^k entries for k >= 7).
 __host__ __device__ constexpr int filter_table_stride(int g) { return g > 1 ? 16384 / g : 0; }
 
-template <int G>
+// SCR: the phase-B form for jobs with the signature screen (see below).
+template <int G, bool SCR>
 __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(FilterArgs a) {
   static_assert(G >= 1 && G <= 16, "one lane of a 16-lane group per representative");
+  static_assert(!SCR || G <= filter_group(kScreenMinK), "the screen covers the long-word groups only");
   constexpr int kTS = filter_table_stride(G);
   HIP_DYNAMIC_SHARED(uint32_t, lds)
   uint16_t* table = reinterpret_cast<uint16_t*>(lds);
@@ -132,6 +149,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
   __shared__ int s_slot[G], s_rep[G], s_n[G];
   __shared__ uint32_t s_kp0[G], s_kn[G];
   __shared__ uint4 s_comp[G];
+  __shared__ int s_act[SCR ? 256 : 1];  // active units of a batch (screened phase B)
   const int tid = threadIdx.x;
   const int nj = a.njobs;
   // prefix of work units over jobs
@@ -159,9 +177,49 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
   const int u0 = blockIdx.x * per;
   const int u1 = min(total, u0 + per);
   const int grp = tid >> 4, gl = tid & 15;
-  int cj = -1, cg = -1, gcount = 0;
+  int ckey = -1, gcount = 0;  // job * 64 + group of the staged tables
   unsigned long long rej_len = 0, rej_comp = 0, rej_word = 0, passed = 0, bytes = 0, cband = 0, cfull = 0;
-  for (int u = u0; u < u1; ++u) {
+  // Phase B with the signature screen (groups of the screened word lengths):
+  // the block checks 256 units at a time, one per thread, and runs only
+  // those where some representative of the unit's group passed the screen
+  // against some candidate of its chunk (the screen has counted the pairs of
+  // the others). Without the screen every unit of the range runs.
+  constexpr bool compact = SCR;
+  for (int ub = u0; ub < u1; ub += compact ? 256 : u1 - u0) {
+    int nact = u1 - u0;
+    if (compact) {
+      const int u = ub + tid;
+      bool act = false;
+      if (u < u1) {
+        int lo = 0, hi = nj - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_pre[mid] <= u) lo = mid; else hi = mid - 1;
+        }
+        const JobDev& J = a.jobs[lo];
+        const JobState* S = J.state;
+        act = true;
+        if (J.scr != nullptr && !(J.flags & kJobDropFilters)) {
+          const int chunks = (S->nc - S->k_spec + filter_chunk(G) - 1) / filter_chunk(G);
+          const int lu = u - s_pre[lo];
+          const int gi = lu / chunks;
+          // the group's slots: the true representatives of ranks gi*G ..
+          unsigned long long tm = S->tmask;
+          for (int i = 0; i < gi * G && tm; ++i) tm &= tm - 1;
+          unsigned long long gslots = 0;
+          for (int g = 0; g < G && tm; ++g) {
+            gslots |= tm & (~tm + 1ull);
+            tm &= tm - 1;
+          }
+          act = (J.chm[lu - gi * chunks] & gslots) != 0;
+        }
+      }
+      int at = block_exclusive_scan<256>(act ? 1 : 0, &nact, s_part);
+      if (act) s_act[at] = u;
+      __syncthreads();
+    }
+  for (int ai = 0; ai < nact; ++ai) {
+    const int u = compact ? s_act[ai] : ub + ai;
     // job of unit u: last j with s_pre[j] <= u
     int lo = 0, hi = nj - 1;
     while (lo < hi) {
@@ -181,7 +239,9 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     const int lu = u - s_pre[job];
     const int gi = lu / chunks;
     const int chunk = lu - gi * chunks;
-    if (job != cj || gi != cg) {
+    // a screened job's pairs that the screen rejected are skipped (counted there)
+    const bool screened = compact && J.scr != nullptr && !(J.flags & kJobDropFilters);
+    if (job * 64 + gi != ckey) {
       // stage the group: slots (phase A: ranks are slots; phase B: the
       // rank-th true representative), records and k-mer tables
       const int nw = 1 << (2 * J.k);
@@ -215,8 +275,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         }
       }
       __syncthreads();
-      cj = job;
-      cg = gi;
+      ckey = job * 64 + gi;
     }
     // every lane of the block runs the whole unit (no early exit): the
     // survivor append below is a wave-wide operation. The NS candidates of a
@@ -226,6 +285,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     // each candidate's record: (m, bc, wc, ksum) and (composition, profile)
     int qv[NS];
     uint4 rav[NS], rbv[NS];
+    unsigned long long svv[NS];  // slots that passed the screen against q
 #pragma unroll
     for (int sub = 0; sub < NS; ++sub) {
       const int ps = p0 + chunk * filter_chunk(G) + sub * kFilterChunk + grp;
@@ -236,6 +296,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       const uint4* rp = reinterpret_cast<const uint4*>(J.st.rec + qv[sub]);
       rav[sub] = rp[0];
       rbv[sub] = rp[1];
+      svv[sub] = screened ? J.scr[qv[sub]] : ~0ull;
     }
 #pragma unroll
     for (int sub = 0; sub < NS; ++sub) {
@@ -252,6 +313,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (!inpos || g >= gcount || pos <= s_slot[g]) continue;
+        if (!((svv[sub] >> s_slot[g]) & 1ull)) continue;
         if (c > min(s_n[g], mq)) {
           if (gl == 0) ++rej_len;
           continue;
@@ -416,6 +478,9 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
       }
     }
   }
+    // the next batch rewrites s_act
+    if (compact) __syncthreads();
+  }
   unsigned long long v[7] = {rej_len, rej_comp, rej_word, passed, bytes, cband, cfull};
   const int lane = tid & 63;
 #pragma unroll
@@ -426,15 +491,215 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
     v[i] = x;
   }
   if (lane == 0 && (v[0] | v[1] | v[2] | v[3])) {
-    atomicAdd(&a.stats[ST_PAIRS], v[0] + v[1] + v[2] + v[3]);
-    if (v[0]) atomicAdd(&a.stats[ST_REJ_LEN], v[0]);
-    if (v[1]) atomicAdd(&a.stats[ST_REJ_COMP], v[1]);
-    if (v[2]) atomicAdd(&a.stats[ST_REJ_WORD], v[2]);
-    if (v[3]) atomicAdd(&a.stats[ST_ALIGNED], v[3]);
-    atomicAdd(&a.stats[ST_FILTER_BYTES], v[4]);
-    if (v[5]) atomicAdd(&a.stats[ST_CELLS_BAND], v[5]);
-    if (v[6]) atomicAdd(&a.stats[ST_CELLS_FULL], v[6]);
+    atomicAdd(&stat_row(a.stats)[ST_PAIRS], v[0] + v[1] + v[2] + v[3]);
+    if (v[0]) atomicAdd(&stat_row(a.stats)[ST_REJ_LEN], v[0]);
+    if (v[1]) atomicAdd(&stat_row(a.stats)[ST_REJ_COMP], v[1]);
+    if (v[2]) atomicAdd(&stat_row(a.stats)[ST_REJ_WORD], v[2]);
+    if (v[3]) atomicAdd(&stat_row(a.stats)[ST_ALIGNED], v[3]);
+    atomicAdd(&stat_row(a.stats)[ST_FILTER_BYTES], v[4]);
+    if (v[5]) atomicAdd(&stat_row(a.stats)[ST_CELLS_BAND], v[5]);
+    if (v[6]) atomicAdd(&stat_row(a.stats)[ST_CELLS_FULL], v[6]);
   }
+}
+
+// ------------------------------------------------------- signature screen
+// Phase B, before the filter, for jobs whose store has signatures (k >= 7):
+// one block per chunk of 64 candidate positions (the filter's phase-B chunk
+// for these word lengths), one candidate per lane, its record and signature
+// in registers of each of the block's four waves. The step's true
+// representatives (records and signatures) are staged in LDS once per job
+// and split over the waves. A pair is rejected here, and counted, when it
+// fails the filter's length or composition test, or when the signature bound
+// exceeds the candidate's slack T_q - wc_q: its shared-word sum is then below
+// wc_q (func.cpp:567) whatever the profiles hold. The slots that survive go
+// to scr[q], their OR over the chunk to chm[chunk]. The count over any part
+// of the signature is a bound too: it runs a quarter of the words at a time
+// and stops once every lane has decided (for unrelated pairs the first
+// quarter already exceeds the slack).
+static_assert(filter_chunk(filter_group(kScreenMinK)) == 64 && filter_chunk(filter_group(8)) == 64,
+              "one block covers one phase-B filter chunk of the screened word lengths");
+
+__global__ __launch_bounds__(256) void screen_kernel(ScreenArgs a) {
+  __shared__ int s_pre[kMaxJobsPerLaunch + 1];
+  __shared__ int s_part[4];
+  __shared__ uint4 s_sig[kMaxSpec][kSigWords / 4];  // representatives by rank
+  __shared__ int s_slot[kMaxSpec], s_m[kMaxSpec];
+  __shared__ uint32_t s_cac[kMaxSpec], s_cgt[kMaxSpec];
+  __shared__ unsigned long long s_sv[4][64], s_cm[4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nj = a.njobs;
+  {
+    int carry = 0;
+    for (int base = 0; base < nj; base += 256) {
+      const int j = base + tid;
+      int u = 0;
+      if (j < nj) {
+        const JobDev& J = a.jobs[j];
+        const JobState* S = J.state;
+        if (J.scr != nullptr && !(J.flags & kJobDropFilters) && !S->done) {
+          const int rest = S->nc - S->k_spec;
+          u = rest > 0 ? (rest + 63) >> 6 : 0;
+        }
+      }
+      int tot;
+      const int ex = block_exclusive_scan<256>(u, &tot, s_part);
+      if (j < nj) s_pre[j] = carry + ex;
+      carry += tot;
+    }
+    if (tid == 0) s_pre[nj] = carry;
+    __syncthreads();
+  }
+  const int total = s_pre[nj];
+  unsigned long long rej_len = 0, rej_comp = 0, rej_sig = 0, cands = 0;
+  int staged = -1, kt = 0;
+  for (int u = blockIdx.x; u < total; u += gridDim.x) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    // the same job for the whole block: scalar loads from here on
+    const int job = __builtin_amdgcn_readfirstlane(lo);
+    const JobDev& J = a.jobs[job];
+    const JobState* S = J.state;
+    const int K = S->k_spec, nc = S->nc;
+    const int32_t* cand = J.cand[S->cur];
+    const unsigned long long tm = S->tmask;
+    if (job != staged) {
+      // stage the true representatives of the step by rank: slot, length,
+      // composition and signature
+      __syncthreads();
+      kt = __popcll(tm);
+      if (tid < 64 && ((tm >> tid) & 1ull)) {
+        const int rank = __popcll(tm & ((1ull << tid) - 1ull));
+        const SeqRec rr = J.st.rec[cand[tid]];
+        s_slot[rank] = tid;
+        s_m[rank] = rr.m;
+        s_cac[rank] = rr.comp_ac;
+        s_cgt[rank] = rr.comp_gt;
+      }
+      __syncthreads();
+      // four threads per representative, 64 bytes each
+      const int rank = tid >> 2, part = tid & 3;
+      if (rank < kt) {
+        const uint4* src = reinterpret_cast<const uint4*>(J.st.sig + (int64_t)cand[s_slot[rank]] * kSigWords);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s_sig[rank][part * 4 + i] = src[part * 4 + i];
+      }
+      __syncthreads();
+      staged = job;
+    }
+    const int chunk = u - s_pre[job];
+    const int pos = K + chunk * 64 + lane;
+    const bool inpos = pos < nc;
+    const int q = inpos ? cand[pos] : cand[K];
+    const uint4* rp = reinterpret_cast<const uint4*>(J.st.rec + q);
+    const uint4 ra = rp[0], rb = rp[1];
+    const int mq = (int)ra.x, bcq = (int)ra.y, c = bcq + 1;
+    const int slack = (int)ra.w - (int)ra.z;  // T_q - wc_q
+    const uint32_t qa = rb.x & 0xFFFFu, qc = rb.x >> 16, qg = rb.y & 0xFFFFu, qt = rb.y >> 16;
+    uint32_t sq[kSigWords];
+    const uint4* sp = reinterpret_cast<const uint4*>(J.st.sig + (int64_t)q * kSigWords);
+#pragma unroll
+    for (int w = 0; w < kSigWords / 4; ++w) {
+      const uint4 v = sp[w];
+      sq[4 * w] = v.x;
+      sq[4 * w + 1] = v.y;
+      sq[4 * w + 2] = v.z;
+      sq[4 * w + 3] = v.w;
+    }
+    if (inpos && wid == 0) ++cands;
+    unsigned long long sv = 0, cm = 0;
+    for (int ri = wid; ri < kt; ri += 4) {
+      const int slot = __builtin_amdgcn_readfirstlane(s_slot[ri]);
+      const int rm = __builtin_amdgcn_readfirstlane(s_m[ri]);
+      const uint32_t cac = __builtin_amdgcn_readfirstlane(s_cac[ri]);
+      const uint32_t cgt = __builtin_amdgcn_readfirstlane(s_cgt[ri]);
+      bool alive = inpos;
+      if (alive && c > min(rm, mq)) {
+        ++rej_len;
+        alive = false;
+      }
+      if (alive) {
+        const uint32_t sum = min(cac & 0xFFFFu, qa) + min(cac >> 16, qc) + min(cgt & 0xFFFFu, qg) + min(cgt >> 16, qt);
+        if ((int)sum <= bcq) {
+          ++rej_comp;
+          alive = false;
+        }
+      }
+      if (alive) {
+        // buckets of q missing from r: a lower bound of the deficit
+        int miss = 0;
+#pragma unroll
+        for (int part = 0; part < 4; ++part) {
+          if (part > 0 && __builtin_amdgcn_ballot_w64(miss <= slack) == 0) break;
+#pragma unroll
+          for (int w = part * kSigWords / 16; w < (part + 1) * kSigWords / 16; ++w) {
+            const uint4 r = s_sig[ri][w];
+            miss += __builtin_popcount(sq[4 * w] & ~r.x) + __builtin_popcount(sq[4 * w + 1] & ~r.y) +
+                    __builtin_popcount(sq[4 * w + 2] & ~r.z) + __builtin_popcount(sq[4 * w + 3] & ~r.w);
+          }
+        }
+        if (miss > slack) {
+          ++rej_sig;
+          alive = false;
+        }
+      }
+      if (alive) sv |= 1ull << slot;
+      if (__builtin_amdgcn_ballot_w64(alive)) cm |= 1ull << slot;
+    }
+    // the waves' survivors of each candidate and of the chunk
+    s_sv[wid][lane] = sv;
+    if (lane == 0) s_cm[wid] = cm;
+    __syncthreads();
+    if (wid == 0) {
+      if (inpos) J.scr[q] = s_sv[0][lane] | s_sv[1][lane] | s_sv[2][lane] | s_sv[3][lane];
+      if (lane == 0) J.chm[chunk] = s_cm[0] | s_cm[1] | s_cm[2] | s_cm[3];
+    }
+    __syncthreads();
+  }
+  unsigned long long v[4] = {rej_len, rej_comp, rej_sig, cands};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    unsigned long long x = v[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[i] = x;
+  }
+  if (lane == 0 && (v[0] | v[1] | v[2] | v[3])) {
+    atomicAdd(&stat_row(a.stats)[ST_PAIRS], v[0] + v[1] + v[2]);
+    if (v[0]) atomicAdd(&stat_row(a.stats)[ST_REJ_LEN], v[0]);
+    if (v[1]) atomicAdd(&stat_row(a.stats)[ST_REJ_COMP], v[1]);
+    if (v[2]) {
+      atomicAdd(&stat_row(a.stats)[ST_REJ_WORD], v[2]);
+      atomicAdd(&stat_row(a.stats)[ST_SCREENED], v[2]);
+    }
+    // id, record and signature of each candidate, its survivor mask
+    if (v[3]) atomicAdd(&stat_row(a.stats)[ST_FILTER_BYTES], v[3] * (4 + 32 + 4 * kSigWords + 8));
+  }
+}
+
+// Signatures of every sequence: one wave per sequence, the 2048 bits built
+// in LDS from the distinct k-mers with a nonzero stored count.
+__global__ __launch_bounds__(256) void signature_kernel(const uint32_t* __restrict__ kp,
+                                                        const int64_t* __restrict__ kpoff, int n, int k,
+                                                        uint32_t* __restrict__ sig) {
+  __shared__ uint32_t bits[4][kSigWords];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 4 + wv;
+  bits[wv][lane] = 0u;
+  __syncthreads();
+  if (s < n) {
+    for (int64_t e = kpoff[s] + lane; e < kpoff[s + 1]; e += 64) {
+      const uint32_t x = kp[e];
+      if (x >> 16) {
+        const uint32_t b = sig_bucket(x & 0xFFFFu, k);
+        atomicOr(&bits[wv][b >> 5], 1u << (b & 31));
+      }
+    }
+  }
+  __syncthreads();
+  if (s < n) sig[(int64_t)s * kSigWords + lane] = bits[wv][lane];
 }
 
 // ------------------------------------------------------- pigeonhole test
@@ -510,8 +775,8 @@ __global__ __launch_bounds__(256) void pigeon_kernel(PigeonArgs a) {
     }
   }
   if (lane == 0 && (checked | rej)) {
-    atomicAdd(&a.stats[ST_PIGEON_CHECKED], checked);
-    if (rej) atomicAdd(&a.stats[ST_REJ_PIGEON], rej);
+    atomicAdd(&stat_row(a.stats)[ST_PIGEON_CHECKED], checked);
+    if (rej) atomicAdd(&stat_row(a.stats)[ST_REJ_PIGEON], rej);
   }
 }
 
@@ -733,6 +998,13 @@ __device__ __forceinline__ void row_masks64(uint32_t (&v)[W], const uint32_t (&t
 // each block end the zero count z gives the exact early decisions z >= c
 // (accept) and z + rows_left < c (reject); the window slides by one lane when
 // the band's left edge has passed the lowest lane.
+// True when the list is already taken: the waves still arriving leave
+// without an atomic on the shared counter (checked once, before a wave's
+// first reservation).
+__device__ __forceinline__ bool work_taken(const int* work, int total) {
+  return __hip_atomic_load(work, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= total;
+}
+
 struct RaggedRows { static constexpr bool value = true; };
 struct FullRows { static constexpr bool value = false; };
 
@@ -759,7 +1031,7 @@ __device__ inline void align_loop(const AlignArgs& a) {
   int64_t toff = 0, qoff = 0;
   unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
   const int total = *a.count;
-  if (total == 0) return;
+  if (total == 0 || work_taken(a.work, total)) return;
 
   while (true) {
     // ---- fetch a pair for every segment without one; pairs decided without
@@ -974,11 +1246,12 @@ __device__ inline void align_loop(const AlignArgs& a) {
     nskip += __shfl_xor(nskip, off);
   }
   if (lane == 0 && a.stats) {
-    if (cells) atomicAdd(&a.stats[ST_CELLS_EXEC], cells);
-    if (cdec) atomicAdd(&a.stats[ST_CELLS_DECIDED], cdec);
-    if (nskip) atomicAdd(&a.stats[ST_SPEC_WASTE], nskip);
-    if (nacc) atomicAdd(&a.stats[ST_ACCEPT], nacc);
-    if (nrej) atomicAdd(&a.stats[ST_REJ_ALIGN], nrej);
+    unsigned long long* sr = stat_row(a.stats);
+    if (cells) atomicAdd(&sr[ST_CELLS_EXEC], cells);
+    if (cdec) atomicAdd(&sr[ST_CELLS_DECIDED], cdec);
+    if (nskip) atomicAdd(&sr[ST_SPEC_WASTE], nskip);
+    if (nacc) atomicAdd(&sr[ST_ACCEPT], nacc);
+    if (nrej) atomicAdd(&sr[ST_REJ_ALIGN], nrej);
   }
 }
 
@@ -1260,6 +1533,7 @@ __device__ inline void align_loop64(const AlignArgs& a) {
   const Stage64 X{stage[wid][0], stage[wid][1], stage[wid][2]};
   unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
   const int total = *a.count;
+  if (total == 0 || work_taken(a.work, total)) return;
   while (true) {
     int got = 0;
     if (lane == 0) got = atomicAdd(a.work, 1);
@@ -1300,11 +1574,12 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     }
   }
   if (lane == 0 && a.stats) {
-    if (cells) atomicAdd(&a.stats[ST_CELLS_EXEC], cells);
-    if (cdec) atomicAdd(&a.stats[ST_CELLS_DECIDED], cdec);
-    if (nskip) atomicAdd(&a.stats[ST_SPEC_WASTE], nskip);
-    if (nacc) atomicAdd(&a.stats[ST_ACCEPT], nacc);
-    if (nrej) atomicAdd(&a.stats[ST_REJ_ALIGN], nrej);
+    unsigned long long* sr = stat_row(a.stats);
+    if (cells) atomicAdd(&sr[ST_CELLS_EXEC], cells);
+    if (cdec) atomicAdd(&sr[ST_CELLS_DECIDED], cdec);
+    if (nskip) atomicAdd(&sr[ST_SPEC_WASTE], nskip);
+    if (nacc) atomicAdd(&sr[ST_ACCEPT], nacc);
+    if (nrej) atomicAdd(&sr[ST_REJ_ALIGN], nrej);
   }
 }
 
@@ -1339,6 +1614,10 @@ hipError_t launch_clamp_spec(const JobDev* jobs, int njobs, long long pair_budge
 // Per sequence, the sum of the counts stored in its k-mer profile.
 hipError_t launch_profile_totals(const uint32_t* kp, const int64_t* kpoff, int n, int32_t* ksum, hipStream_t s);
 hipError_t launch_pigeon(const PigeonArgs& a, int grid, hipStream_t s);
+// Signature screen of phase B (jobs with signatures), before the filter.
+hipError_t launch_screen(const ScreenArgs& a, int grid, hipStream_t s);
+// Word-presence signatures of n sequences (kSigWords words each).
+hipError_t launch_signatures(const uint32_t* kp, const int64_t* kpoff, int n, int k, uint32_t* sig, hipStream_t s);
 // Cutoffs of n sequences for threshold t and word length k (func.cpp:339-374).
 // (also into the records' bc and wc when rec is not null)
 hipError_t launch_cutoffs(const int32_t* raw_len, int n, float t, int k, int32_t* bc, int32_t* wc, SeqRec* rec,

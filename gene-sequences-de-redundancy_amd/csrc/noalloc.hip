@@ -26,10 +26,11 @@ namespace {
 
 // Layout of the array: job table, shape counters, statistics, running-jobs
 // counter, then the kNumShapes alignment lists of equal capacity.
-constexpr size_t kJobsOff = 0, kCountersOff = 256, kStatsOff = 512, kLeftOff = 768, kItemsOff = 1024;
+constexpr size_t kJobsOff = 0, kCountersOff = 256, kStatsOff = 512;
+constexpr size_t kLeftOff = kStatsOff + kStatWords * sizeof(unsigned long long), kItemsOff = kLeftOff + 256;
 static_assert(sizeof(JobDev) <= kCountersOff - kJobsOff, "job table slot");
 static_assert(2 * kNumShapes * sizeof(int32_t) <= kStatsOff - kCountersOff, "counter slot");
-static_assert(ST_NUM * sizeof(unsigned long long) <= kLeftOff - kStatsOff, "statistics slot");
+static_assert(kStatWords * sizeof(unsigned long long) <= kLeftOff - kStatsOff, "statistics slot");
 
 __device__ __align__(256) unsigned char scratch[GSDR_NOALLOC_SCRATCH_BYTES];
 

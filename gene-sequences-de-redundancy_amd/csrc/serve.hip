@@ -87,10 +87,12 @@ static hipError_t capture_steps(gsdr_server* srv, const StepLaunch& L, hipStream
 }
 
 // Upper bound of a job's HBM use: packed store, k-mer profile (at most one
-// entry per base), per-sequence arrays and round state.
-static int64_t footprint_bound(const Records& r) {
+// entry per base), per-sequence arrays and round state, the signature screen's
+// arrays for long words.
+static int64_t footprint_bound(const Records& r, int k) {
   const int64_t n = r.count(), b = (int64_t)r.data.size();
-  return b / 4 + 4 * b + n * 120 + (1 << 16);
+  const int64_t screen = k >= kScreenMinK ? n * (4 * kSigWords + 8) + n / 8 + 1024 : 0;
+  return b / 4 + 4 * b + n * 120 + screen + (1 << 16);
 }
 
 extern "C" {
@@ -109,7 +111,7 @@ int gsdr_serve_open(gsdr_ctx* c, int64_t hbm_budget, int32_t max_slots, gsdr_ser
   if (st) return st;
   hipError_t e;
   if ((e = hipMemsetAsync(c->ws.counters, 0, 2 * kNumShapes * sizeof(int32_t), c->stream)) != hipSuccess ||
-      (e = hipMemsetAsync(c->ws.stats, 0, ST_NUM * sizeof(unsigned long long), c->stream)) != hipSuccess ||
+      (e = hipMemsetAsync(c->ws.stats, 0, kStatWords * sizeof(unsigned long long), c->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return hip_fail(e, "serve init");
   *srv = s.release();
@@ -127,7 +129,7 @@ int gsdr_serve_submit(gsdr_server* srv, const gsdr_records* sorted, const gsdr_p
   if (!(t >= 0.8f) || t >= 1.0f) return fail(GSDR_E_INVALID, "similarity out of range: 0.8<=similarity<1");
   const int k = params->word_length ? params->word_length : auto_word_length(t);
   if (k < 4 || k > 8) return fail(GSDR_E_INVALID, "word length out of range: 4<=word length<=8");
-  const int64_t need = footprint_bound(r);
+  const int64_t need = footprint_bound(r, k);
   auto job = std::make_unique<ServeJob>();
   job->t_submit = Clock::now();
   {

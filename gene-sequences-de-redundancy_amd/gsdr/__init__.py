@@ -61,12 +61,11 @@ class Stats(C.Structure):
         ("cells_decided", C.c_double),
         ("rej_pigeon", C.c_double),
         ("pigeon_checked", C.c_double),
-        ("reserved", C.c_double * 1),
+        ("screened", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: (getattr(self, name) if name != "reserved" else None)
-                for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Tuning(C.Structure):
@@ -79,9 +78,15 @@ class Tuning(C.Structure):
         ("fork_shapes", C.c_int32),
         ("list_factor", C.c_int32),
         ("pigeon", C.c_int32),
+        ("screen", C.c_int32),
         ("ms_default", C.c_double),
         ("ms_tuned", C.c_double),
     ]
+
+    def __init__(self, **kw):
+        # the signature screen is on unless a caller turns it off
+        kw.setdefault("screen", 1)
+        super().__init__(**kw)
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -325,7 +330,9 @@ class Engine:
                                       len(pr), shape, _i32p(out), C.byref(cells)), "align_pairs")
         return out.astype(bool), cells.value
 
-    def cluster_jobs(self, stores, thresholds, mode=MODE_BATCHED, spec_reps=0):
+    def cluster_jobs(self, stores, thresholds, mode=MODE_BATCHED, spec_reps=0, drop_filters=None):
+        """Clusters several stores in shared launches; drop_filters: one flag
+        per job (the drop mode), or None."""
         n = len(stores)
         params = (Params * n)()
         for i, t in enumerate(thresholds):
@@ -333,6 +340,7 @@ class Engine:
             params[i].word_length = stores[i].k if hasattr(stores[i], "k") else 0
             params[i].mode = mode
             params[i].spec_reps = spec_reps
+            params[i].drop_filters = 1 if drop_filters is not None and drop_filters[i] else 0
         outs = [np.zeros(s.n, dtype=np.int32) for s in stores]
         handles = (C.c_void_p * n)(*[s.h.value for s in stores])
         ptrs = (C.POINTER(C.c_int32) * n)(*[_i32p(o) for o in outs])

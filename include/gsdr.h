@@ -84,7 +84,7 @@ typedef struct gsdr_stats {
   double rej_pigeon;    /* filtered pairs rejected by the exact pigeonhole test (tuning.pigeon = 1);
                            aligned = accepted + rej_align + spec_wasted + rej_pigeon */
   double pigeon_checked; /* filtered pairs the pigeonhole test examined */
-  double reserved[1];
+  double screened;      /* pairs the word-signature screen rejected (k >= 6; part of rej_word) */
 } gsdr_stats;
 
 /* Launch knobs that change speed, never results. */
@@ -99,6 +99,7 @@ typedef struct gsdr_tuning {
   int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
                              sequence (default 64); it also bounds k_spec * candidates */
   int32_t pigeon;         /* 1 = exact pigeonhole prefilter before the alignment (sound; default 0) */
+  int32_t screen;         /* 1 = word-signature screen before the filter for k >= 6 (sound; default 1) */
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
 } gsdr_tuning;
