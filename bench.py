@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counters)")
     ap.add_argument("--tune", action="store_true", help="autotune the launch knobs on the workload first")
-    ap.add_argument("--tune-trials", type=int, default=12)
+    ap.add_argument("--tune-trials", type=int, default=40)
     ap.add_argument("--pigeon", action="store_true", help="enable the exact pigeonhole prefilter")
     ap.add_argument("--no-screen", action="store_true", help="turn the word-signature screen off")
     ap.add_argument("--serial-shapes", action="store_true",

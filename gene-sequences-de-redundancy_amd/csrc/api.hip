@@ -358,8 +358,13 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
   if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || t->filter_blocks < 1 ||
       t->filter_blocks > 64 || t->align_blocks < 0 || t->align_blocks > 64 ||
       (t->fork_shapes != 0 && t->fork_shapes != 1) || t->list_factor < 1 || t->list_factor > 256 ||
-      (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1))
+      (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1) || t->win_period < 0 ||
+      t->win_period > 16)
     return fail(GSDR_E_INVALID, "tuning out of range");
+  for (int b = 0; b < kLenBins; ++b)
+    if (t->shape_mask[b] < 0 || (uint32_t)t->shape_mask[b] > kAllShapes || t->shape_bias[b] < 0 ||
+        t->shape_bias[b] > 3)
+      return fail(GSDR_E_INVALID, "tuning out of range");
   Tuning T;
   T.swap_ratio = t->swap_ratio;
   T.spec_max = t->spec_max;
@@ -369,6 +374,11 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
   T.list_factor = t->list_factor;
   T.pigeon = t->pigeon;
   T.screen = t->screen;
+  for (int b = 0; b < kLenBins; ++b) {
+    T.shapes.mask[b] = (uint16_t)t->shape_mask[b];
+    T.shapes.bias[b] = (uint8_t)t->shape_bias[b];
+  }
+  T.win_period = t->win_period;
   c->tuned = T;
   c->has_tuning = true;
   return GSDR_OK;
@@ -386,6 +396,7 @@ int gsdr_get_tuning(const gsdr_ctx* c, gsdr_tuning* out) {
   out->list_factor = T.list_factor;
   out->pigeon = T.pigeon;
   out->screen = T.screen;
+  tuning_shapes_out(T, out);
   return GSDR_OK;
 }
 

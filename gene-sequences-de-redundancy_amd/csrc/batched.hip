@@ -63,6 +63,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.fa_main.njobs = njobs;
   L.fa_main.min_shape = 0;
   L.fa_main.swap_ratio = T.swap_ratio;
+  L.fa_main.shapes = T.shapes;
   L.fa_main.phase = PHASE_MAIN;
   L.fa_main.items = w.items;
   L.fa_main.item_cap = w.item_cap;
@@ -85,6 +86,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
     aa.result = nullptr;
     aa.stats = w.stats;
     aa.skip_absorbed = 1;
+    aa.win_period = T.win_period;
     L.aa_reps[sh] = aa;
     L.aa_reps[sh].skip_absorbed = 0;
   }

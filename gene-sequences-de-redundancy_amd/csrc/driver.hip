@@ -246,6 +246,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     fa.njobs = 1;
     fa.min_shape = 0;
     fa.swap_ratio = T.swap_ratio;
+    fa.shapes = T.shapes;
     fa.phase = PHASE_MAIN;
     fa.items = w.items;
     fa.item_cap = w.item_cap;
@@ -279,6 +280,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       aa[sh].result = nullptr;
       aa[sh].stats = w.stats;
       aa[sh].skip_absorbed = 0;
+      aa[sh].win_period = T.win_period;
       const int segs = 64 / kShapes[sh].s;
       const int waves = (nc + segs - 1) / segs;
       grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh)));

@@ -40,6 +40,8 @@ struct Tuning {
   int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
   int pigeon = 0;        // exact pigeonhole prefilter between the filter and the alignment
   int screen = 1;        // word-signature screen before the phase-B filter (k >= 6)
+  gsdr::ShapeTable shapes = {};  // tile shapes per text-length bin (zero: the narrowest that holds the band)
+  int win_period = 0;    // 64-lane tiles: blocks between DP-edge window rebases (0 = by width)
 };
 
 struct gsdr_ctx {
@@ -205,6 +207,14 @@ inline int align_blocks_for(const Tuning& T, int shape) {
 }
 // The knobs a call runs with: the context's tuned ones when asked for and present.
 inline Tuning tuning_for(const gsdr_ctx* c, bool want) { return want && c->has_tuning ? c->tuned : Tuning(); }
+// The shape tables and window period of T, as the C ABI reports them.
+inline void tuning_shapes_out(const Tuning& T, gsdr_tuning* out) {
+  for (int b = 0; b < gsdr::kLenBins; ++b) {
+    out->shape_mask[b] = T.shapes.mask[b];
+    out->shape_bias[b] = T.shapes.bias[b];
+  }
+  out->win_period = T.win_period;
+}
 int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int max_trials, gsdr_tuning* out);
 hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev);
 }  // namespace gsdr

@@ -16,18 +16,49 @@ struct Shape { int s, w; };
 constexpr Shape kShapes[kNumShapes] = {{8, 1},  {16, 1}, {32, 1}, {64, 1},  {64, 2},  {64, 3},
                                        {64, 4}, {64, 6}, {64, 8}, {64, 10}, {64, 12}, {64, 16}};
 
-// Smallest shape that holds a pair with text length n, query length m and
-// minimum accepted LCS c (both gap-free lengths). E = n + m - 2c bounds the
-// band width; the window slides at 32-row block boundaries, hence +31.
-__host__ __device__ inline int pick_shape(int n, int m, int c, int min_shape) {
+// Tile-shape choice per bin of text length (gap-free text length < 512,
+// < 2,048, < 8,192, longer): the shapes a pair held by its band may take
+// (bit i = shape i; 0 = all) and how many allowed shapes past the narrowest
+// one to go. A shape that holds the whole text is always allowed and always
+// ends the search, so no choice goes past the text's whole-text shape (the
+// bound the launches use). Every choice is exact: a wider shape also holds
+// the band, and any window that holds the band gives the same decision.
+constexpr int kLenBins = 4;
+constexpr uint32_t kAllShapes = (1u << kNumShapes) - 1u;
+struct ShapeTable {
+  uint16_t mask[kLenBins];
+  uint8_t bias[kLenBins];
+};
+__host__ __device__ inline int len_bin(int n) { return n < 512 ? 0 : n < 2048 ? 1 : n < 8192 ? 2 : 3; }
+
+// Shape for a pair with text length n, query length m and minimum accepted
+// LCS c (both gap-free lengths): the narrowest allowed shape that holds it,
+// then `bias` allowed shapes further. E = n + m - 2c bounds the band width;
+// the window slides at 32-row block boundaries, hence +31. A shape holds
+// the pair when its S*W words cover the whole text, or when its sliding
+// window ((S-1)*W words plus one lane of lag) covers the band; both sizes
+// grow along kShapes, so every later shape holds it too.
+__host__ __device__ inline int pick_shape(int n, int m, int c, int min_shape, uint32_t mask = kAllShapes,
+                                          int bias = 0) {
   const int ntw = (n >> 5) + 1;
   const int e = n + m - 2 * c;
   const int need = ((e + 31) >> 5) + 1;
+  if (mask == 0u) mask = kAllShapes;
+  int left = -1;  // allowed shapes still to pass once the band is held
   for (int i = min_shape; i < kNumShapes; ++i) {
     const int s = kShapes[i].s, w = kShapes[i].w;
-    if (s * w >= ntw || (s - 1) * w >= need) return i;
+    if (s * w >= ntw) return i;
+    if (!((mask >> i) & 1u)) continue;
+    if (left < 0 && (s - 1) * w >= need) left = bias;
+    if (left == 0) return i;
+    if (left > 0) --left;
   }
   return kNumShapes - 1;
+}
+
+__host__ __device__ inline int pick_shape(int n, int m, int c, int min_shape, const ShapeTable& t) {
+  const int b = len_bin(n);
+  return pick_shape(n, m, c, min_shape, t.mask[b], t.bias[b]);
 }
 
 // Cells of the exact decision band: rows j = 1..m, text columns

@@ -18,6 +18,7 @@ struct FilterArgs {
   int32_t* bin_count;       // kNumShapes counters
   unsigned long long* stats;
   int32_t screen;           // phase B: skip what the signature screen rejected
+  ShapeTable shapes = {};   // tile-shape choice per text-length bin (zero: narrowest of all)
 };
 
 struct ScreenArgs {
@@ -43,6 +44,8 @@ struct AlignArgs {
   int32_t* result;          // non-null: result[item] = 1 accept / 0 reject (pair-level entry)
   unsigned long long* stats;
   int32_t skip_absorbed;    // skip (slot, q) when an earlier true representative already accepted q
+  int32_t win_period = 0;   // 64-lane tiles: 32-row blocks between DP-edge window rebases (0 = 2 for
+                            // 6 or more words per lane, 4 for fewer)
 };
 
 struct ResolveArgs {
@@ -447,7 +450,7 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         cband += (unsigned long long)band_cells(n, mq, c);
         cfull += (unsigned long long)n * (unsigned long long)mq;
         const bool sw = swap_roles(n, mq, a.swap_ratio);
-        sh = sw ? pick_shape(mq, n, c, a.min_shape) : pick_shape(n, mq, c, a.min_shape);
+        sh = sw ? pick_shape(mq, n, c, a.min_shape, a.shapes) : pick_shape(n, mq, c, a.min_shape, a.shapes);
         it.text = sw ? q : r;
         it.query = sw ? r : q;
         it.job = job;
@@ -1293,6 +1296,7 @@ struct Pair64 {
   const uint2* pk;
   int64_t toff, qoff;  // first packed word of the bit-vector (text) and row (query) sequences
   int n, mq, c, ntw;   // text length, row count, cutoff + 1, text words
+  int every;           // blocks between DP-edge window rebases (0 = by width)
 };
 
 // Per-wave LDS staging used to re-lay a window.
@@ -1460,8 +1464,8 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     const int last = hi_col >> 5;
     int lo_col = max(0, jn - (mq - c));
     const bool covered = b + 64 * W > last;
-    constexpr int kEvery = W >= 6 ? 2 : 4;
-    if (!covered || (W > 2 && jb % kEvery == 0)) {
+    const int every = P.every > 0 ? P.every : (W >= 6 ? 2 : 4);
+    if (!covered || (W > 2 && jb % every == 0)) {
       // inclusive prefix over the lanes: D at the end of each lane's words;
       // lanes whose last column has D + R < c are dead (a prefix of lanes)
       uint32_t pre = x & 0xffffu;
@@ -1555,7 +1559,7 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     }
     int res = 0;
     if (c <= min(n, mq)) {
-      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1};
+      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1, a.win_period};
       // a step ends when its longest pair does: waves on long pairs get the
       // SIMD's issue slots first, the short pairs fill the gaps
       if (mq >= 16384) __builtin_amdgcn_s_setprio(3);

@@ -81,6 +81,9 @@ class Tuning(C.Structure):
         ("screen", C.c_int32),
         ("ms_default", C.c_double),
         ("ms_tuned", C.c_double),
+        ("shape_mask", C.c_int32 * 4),
+        ("shape_bias", C.c_int32 * 4),
+        ("win_period", C.c_int32),
     ]
 
     def __init__(self, **kw):
@@ -89,7 +92,11 @@ class Tuning(C.Structure):
         super().__init__(**kw)
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if name in ("shape_mask", "shape_bias") else v
+        return out
 
 
 class Options(C.Structure):

@@ -102,6 +102,16 @@ typedef struct gsdr_tuning {
   int32_t screen;         /* 1 = word-signature screen before the filter for k >= 6 (sound; default 1) */
   double ms_default;      /* autotune: clustering time with the default knobs */
   double ms_tuned;        /* autotune: clustering time with the chosen knobs */
+  /* Tile shapes per bin of text length (gap-free bases < 512, < 2,048, < 8,192, longer; the
+   * shapes, in order: 8x1 16x1 32x1 64x1 64x2 64x3 64x4 64x6 64x8 64x10 64x12 64x16, lanes x
+   * 32-bit words per lane). shape_mask[b]: the shapes (bit i = shape i) a pair of bin b whose
+   * band a shape's window holds may take, 0 = all; shape_bias[b]: allowed shapes to go past
+   * the narrowest that holds the band (0..3). A shape that holds the whole text is always
+   * allowed and ends the search. */
+  int32_t shape_mask[4];
+  int32_t shape_bias[4];
+  int32_t win_period;     /* 64-lane tiles: 32-row blocks between window rebases at the DP edge
+                             (0 = 2 with 6 or more words per lane, 4 with fewer; else 1..16) */
 } gsdr_tuning;
 
 /* ---- options (host side): replaces checkOption (func.cpp:6-57) and the
@@ -217,7 +227,7 @@ int gsdr_cluster_jobs(gsdr_ctx *ctx, int32_t njobs, gsdr_store *const *stores,
                       const gsdr_params *params, int32_t *const *cluster_out, gsdr_stats *stats);
 /* Autotuner (no counterpart in the reference): clusters `store` with params
  * (forced to the batched form) under coordinate descent over the knobs of
- * gsdr_tuning, at most max_trials clusterings (0 = 20), and keeps the fastest
+ * gsdr_tuning, at most max_trials clusterings (0 = 40), and keeps the fastest
  * set in the context; later calls with params.tune = 1 use it. Every trial
  * must return the same clusters, else GSDR_E_DEVICE. out (may be NULL) gets
  * the chosen knobs and timings. */

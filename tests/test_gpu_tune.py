@@ -41,6 +41,40 @@ def test_every_knob_setting_gives_the_oracle_clusters(engine, tmp_path, law, n, 
         store.free()
 
 
+# shape masks over the twelve tile shapes (8x1 16x1 32x1 64x1 64x2 64x3 64x4
+# 64x6 64x8 64x10 64x12 64x16): powers of two only, no tile narrower than a
+# wave, everything but the 64x2 tile
+ALL = (1 << 12) - 1
+POW2 = ALL & ~((1 << 5) | (1 << 7) | (1 << 9) | (1 << 10))
+NO_NARROW = ALL & ~7
+
+
+@pytest.mark.parametrize("law,n,t", [(4, 4000, 0.95), (2, 3000, 0.90), (1, 2000, 0.80)])
+def test_shape_tables_give_the_oracle_clusters(engine, tmp_path, law, n, t):
+    """Per-length-bin tile shapes and the window's rebase period change which
+    kernel runs a pair and how wide its window is, never a decision."""
+    r, ocl = _oracle(tmp_path, gsdr.Records.generate(law, 1100 + law, n), t)
+    store = engine.load(r, gsdr.auto_word_length(t))
+    try:
+        tables = [([POW2] * 4, [0] * 4, 0), ([NO_NARROW] * 4, [1] * 4, 1), ([0] * 4, [3] * 4, 8),
+                  ([ALL & ~(1 << 4), POW2, NO_NARROW, 0], [2, 0, 1, 3], 16), ([0] * 4, [0] * 4, 1)]
+        for masks, biases, period in tables:
+            tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1,
+                             list_factor=64, win_period=period)
+            for b in range(4):
+                tu.shape_mask[b] = masks[b]
+                tu.shape_bias[b] = biases[b]
+            engine.set_tuning(tu)
+            got = engine.tuning()
+            assert list(got.shape_mask) == masks and list(got.shape_bias) == biases and got.win_period == period
+            for mode in (gsdr.MODE_PLAIN, gsdr.MODE_BATCHED):
+                cl, st = store.cluster(t, store.k, mode=mode, tune=True)
+                assert np.array_equal(cl, ocl), (masks, biases, period, mode)
+    finally:
+        engine.set_tuning(None)
+        store.free()
+
+
 def test_autotune_then_tuned_calls_match(engine, tmp_path):
     r, ocl = _oracle(tmp_path, gsdr.Records.generate(2, 1002, 3000), 0.90)
     store = engine.load(r, 5)
@@ -51,6 +85,8 @@ def test_autotune_then_tuned_calls_match(engine, tmp_path):
         got = engine.tuning()
         assert (got.swap_ratio, got.spec_max, got.filter_blocks, got.align_blocks) == (
             tu.swap_ratio, tu.spec_max, tu.filter_blocks, tu.align_blocks)
+        assert list(got.shape_mask) == list(tu.shape_mask) and list(got.shape_bias) == list(tu.shape_bias)
+        assert got.win_period == tu.win_period
         cl, _ = store.cluster(0.90, 5, mode=gsdr.MODE_BATCHED, tune=True)
         assert np.array_equal(cl, ocl)
     finally:
@@ -62,10 +98,14 @@ def test_autotune_then_tuned_calls_match(engine, tmp_path):
 
 def test_tuning_out_of_range_is_rejected(engine):
     for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0),
-                dict(align_blocks=65), dict(align_blocks=-1), dict(fork_shapes=2), dict(list_factor=0)]:
+                dict(align_blocks=65), dict(align_blocks=-1), dict(fork_shapes=2), dict(list_factor=0),
+                dict(win_period=17), dict(win_period=-1), dict(shape_mask=4096), dict(shape_bias=4)]:
         tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1, list_factor=64)
         for k, v in bad.items():
-            setattr(tu, k, v)
+            if k in ("shape_mask", "shape_bias"):
+                getattr(tu, k)[2] = v
+            else:
+                setattr(tu, k, v)
         with pytest.raises(gsdr.GsdrError):
             engine.set_tuning(tu)
     d = engine.tuning()
