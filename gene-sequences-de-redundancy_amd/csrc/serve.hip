@@ -228,7 +228,34 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
         hipSuccess)
       return hip_fail(e, "stats copy");
     if ((e = hipStreamSynchronize(q)) != hipSuccess) return hip_fail(e, "serve sync");
-    if (overflow) return fail(GSDR_E_LIMIT, "alignment list overflow");
+    if (overflow) {
+      // A step queued more pairs than a shape list holds (the clamp after
+      // every step keeps k_spec * nc within the lists, so this is not
+      // expected): the running jobs' round state is partly written. They
+      // fail (poll reports GSDR_E_LIMIT) and release their memory, the
+      // counters are cleared, and the server goes on with later jobs.
+      if ((e = hipMemsetAsync(w.stats + ST_OVERFLOW, 0, sizeof(unsigned long long), q)) != hipSuccess ||
+          (e = hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q)) != hipSuccess ||
+          (e = hipStreamSynchronize(q)) != hipSuccess)
+        return hip_fail(e, "overflow reset");
+      drop_graph(srv);
+      for (ServeJob* j : srv->running) {
+        j->t_done = Clock::now();
+        j->error = "alignment list overflow";
+        store_release(*j->store);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->used -= j->reserved;
+        srv->seqs -= j->store->n;
+        j->reserved = 0;
+        j->status = 3;
+      }
+      srv->running.clear();
+      if (running_out) {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        *running_out = (int32_t)srv->waiting.size();
+      }
+      return fail(GSDR_E_LIMIT, "alignment list overflow: the running jobs failed, the server goes on");
+    }
     std::vector<ServeJob*> still;
     for (int i = 0; i < nj; ++i) {
       ServeJob* j = srv->running[i];
@@ -265,6 +292,13 @@ int gsdr_serve_poll(gsdr_server* srv, int64_t job_id, int32_t* cluster_out, gsdr
   auto it = srv->jobs.find(job_id);
   if (it == srv->jobs.end()) return fail(GSDR_E_NOTFOUND, "unknown job id");
   ServeJob* j = it->second.get();
+  if (j->status == 3) {
+    // failed in a step (its memory is already released)
+    *done = 1;
+    const std::string why = j->error;
+    srv->jobs.erase(it);
+    return fail(GSDR_E_LIMIT, why.c_str());
+  }
   if (j->status != 2) {
     *done = 0;
     return GSDR_OK;

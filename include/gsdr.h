@@ -254,10 +254,14 @@ int gsdr_serve_open(gsdr_ctx *ctx, int64_t hbm_budget, int32_t max_slots, gsdr_s
 int gsdr_serve_submit(gsdr_server *srv, const gsdr_records *sorted, const gsdr_params *params,
                       int64_t *job_id);
 /* Advances every admitted job by one persistent step. Returns the number
- * of jobs still running in *running. */
+ * of jobs still running in *running. Should a step overflow the shared
+ * alignment lists (the width clamp after every step is there to prevent it),
+ * the jobs in the device table fail, their memory is released, the call
+ * returns GSDR_E_LIMIT and the server goes on with the jobs admitted later. */
 int gsdr_serve_step(gsdr_server *srv, int32_t *running);
 /* *done = 1 when finished; then cluster_out (n entries) and stats are filled
- * and the job's device memory is released. */
+ * and the job's device memory is released. A job that failed in a step is
+ * reported once with *done = 1 and GSDR_E_LIMIT. */
 int gsdr_serve_poll(gsdr_server *srv, int64_t job_id, int32_t *cluster_out, gsdr_stats *stats,
                     int32_t *done);
 void gsdr_serve_close(gsdr_server *srv);
