@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--tune-trials", type=int, default=40)
     ap.add_argument("--pigeon", action="store_true", help="enable the exact pigeonhole prefilter")
     ap.add_argument("--no-screen", action="store_true", help="turn the word-signature screen off")
+    ap.add_argument("--split-rows", type=int, default=-1,
+                    help="phase-A pairs with this many rows or more on two waves (0 = never; default: built in)")
     ap.add_argument("--serial-shapes", action="store_true",
                     help="run the tile-shape kernels of a step phase one after another (profiling)")
     ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
@@ -345,6 +347,12 @@ def main():
         tu.screen = 0
         eng.set_tuning(tu)
         tuning = dict(tuning or {}, screen=0)
+    if args.split_rows >= 0:
+        # the two-wave (forward and backward halves) phase-A pairs
+        tu = eng.tuning()
+        tu.split_rows = args.split_rows
+        eng.set_tuning(tu)
+        tuning = dict(tuning or {}, split_rows=args.split_rows)
     if args.serial_shapes:
         # the tile-shape kernels of a step phase one after another on one
         # stream (profiling: kernel durations then do not overlap)

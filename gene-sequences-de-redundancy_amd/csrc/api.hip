@@ -262,7 +262,7 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
   HIP_TRY(hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * 4, hipHostMallocDefault), "hipHostMalloc");
   c->h_state_cap = 4;
-  for (int i = 0; i < kNumShapes; ++i) {
+  for (int i = 0; i <= kNumShapes; ++i) {
     HIP_TRY(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming), "hipEventCreate");
   }
@@ -278,7 +278,7 @@ void gsdr_close(gsdr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   workspace_release(c);
-  for (int i = 0; i < kNumShapes; ++i) {
+  for (int i = 0; i <= kNumShapes; ++i) {
     if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
     if (c->join[i]) (void)hipEventDestroy(c->join[i]);
   }
@@ -359,7 +359,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
       t->filter_blocks > 64 || t->align_blocks < 0 || t->align_blocks > 64 ||
       (t->fork_shapes != 0 && t->fork_shapes != 1) || t->list_factor < 1 || t->list_factor > 256 ||
       (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1) || t->win_period < 0 ||
-      t->win_period > 16)
+      t->win_period > 16 || (t->win_period & (t->win_period - 1)) != 0 || (t->split_rows != 0 && (t->split_rows < 64 || t->split_rows > 32768)))
     return fail(GSDR_E_INVALID, "tuning out of range");
   for (int b = 0; b < kLenBins; ++b)
     if (t->shape_mask[b] < 0 || (uint32_t)t->shape_mask[b] > kAllShapes || t->shape_bias[b] < 0 ||
@@ -379,6 +379,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
     T.shapes.bias[b] = (uint8_t)t->shape_bias[b];
   }
   T.win_period = t->win_period;
+  T.split_rows = t->split_rows;
   c->tuned = T;
   c->has_tuning = true;
   return GSDR_OK;

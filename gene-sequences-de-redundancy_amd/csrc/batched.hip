@@ -30,7 +30,9 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   if ((e = launch_filter(L.fa_reps, L.kmask, L.reps_filter_grid, q)) != hipSuccess) return e;
   if (L.pigeon && (e = launch_pigeon(L.pa, L.pigeon_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[1], q)) != hipSuccess) return e;
-  if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q, L.fork)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa_reps, L.smax, L.reps_grid, q, L.fork, L.split ? &L.aa_split : nullptr,
+                               L.split_grid)) != hipSuccess)
+    return e;
   if (ev && (e = hipEventRecord(ev[2], q)) != hipSuccess) return e;
   if ((e = launch_resolve_reps(L.ra, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], q)) != hipSuccess) return e;
@@ -77,12 +79,27 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.fa_reps = L.fa_main;
   L.fa_reps.phase = PHASE_REPS;
   L.fa_reps.screen = 0;
+  L.split = T.split_rows > 0;
+  L.fa_reps.split_rows = T.split_rows;
+  // the split kernel's resident blocks: 4 per CU (two waves each, at most
+  // two waves per SIMD by its registers); the split list takes one pair per
+  // block, so every split pair starts at once
+  L.split_grid = c->cu_count * 4;
+  L.fa_reps.split_cap = std::min<int64_t>(w.split_cap, L.split_grid);
+  L.aa_split.jobs = w.jobs;
+  L.aa_split.items = w.items + (int64_t)kSplitList * w.item_cap;
+  L.aa_split.count = w.counters + kSplitList;
+  L.aa_split.work = w.counters + kWorkCounter0 + kSplitList;
+  L.aa_split.result = nullptr;
+  L.aa_split.stats = w.stats;
+  L.aa_split.skip_absorbed = 0;
+  L.aa_split.win_period = T.win_period;
   for (int sh = 0; sh < kNumShapes; ++sh) {
     AlignArgs& aa = L.aa[sh];
     aa.jobs = w.jobs;
     aa.items = w.items + (int64_t)sh * w.item_cap;
     aa.count = w.counters + sh;
-    aa.work = w.counters + kNumShapes + sh;
+    aa.work = w.counters + kWorkCounter0 + sh;
     aa.result = nullptr;
     aa.stats = w.stats;
     aa.skip_absorbed = 1;
@@ -93,7 +110,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.ra.jobs = w.jobs;
   L.ra.njobs = njobs;
   L.ra.counters = w.counters;
-  L.ra.ncounters = 2 * kNumShapes;
+  L.ra.ncounters = kNumCounters;
   L.ra.jobs_left = jobs_left;
   L.ra.stats = w.stats;
   L.ra.pair_budget = w.item_cap;
@@ -164,7 +181,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   for (int j = 0; j < njobs; ++j) left += jobs[j].n > 0;
   HIP_TRY(hipMemcpyAsync(w.jobs, jobs, sizeof(JobDev) * njobs, hipMemcpyHostToDevice, q), "job table");
   HIP_TRY(hipMemsetAsync(w.stats, 0, kStatWords * sizeof(unsigned long long), q), "memset");
-  HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.counters, 0, kNumCounters * sizeof(int32_t), q), "memset");
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs, njobs, k_max, w.jobs_left, q), "init");
 
@@ -179,8 +196,8 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   const int64_t step_limit = max_n + 2;
   // kernels per step: two filters, resolve_reps and the three resolve
   // kernels, the shape kernels of both phases, the clamp with several jobs,
-  // the screen and the pigeonhole tests when on
-  const int64_t per_step = 6 + 2 * (L.smax + 1) + (njobs > 1 ? 1 : 0) + L.screen + 2 * L.pigeon;
+  // the screen and the pigeonhole tests when on, the split kernel of phase A
+  const int64_t per_step = 6 + 2 * (L.smax + 1) + (njobs > 1 ? 1 : 0) + L.screen + 2 * L.pigeon + L.split;
   if (timing) {
     // timed form: the same step launched directly with events around the
     // filter and the alignment kernels

@@ -85,8 +85,10 @@ int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs) {
   int64_t F = 0;
   int st;
   const int64_t cap = std::max<int64_t>(item_cap, 1024);
-  if ((st = dev_alloc(A, F, &w.items, (size_t)cap * kNumShapes)) ||
-      (st = dev_alloc(A, F, &w.counters, (size_t)2 * kNumShapes)) ||
+  // the split list follows the shape lists: every phase-A pair of every job
+  const int64_t split_cap = (int64_t)kSplitPerJob * std::max(njobs, 1);
+  if ((st = dev_alloc(A, F, &w.items, (size_t)cap * kNumShapes + (size_t)split_cap)) ||
+      (st = dev_alloc(A, F, &w.counters, (size_t)kNumCounters)) ||
       (st = dev_alloc(A, F, &w.stats, (size_t)kStatWords)) ||
       (st = dev_alloc(A, F, &w.jobs, (size_t)std::max(njobs, 1))) || (st = dev_alloc(A, F, &w.jobs_left, (size_t)1))) {
     for (void* p : A) (void)hipFree(p);
@@ -94,6 +96,7 @@ int workspace_reserve(gsdr_ctx* c, int64_t item_cap, int njobs) {
     return st;
   }
   w.item_cap = cap;
+  w.split_cap = split_cap;
   w.jobs_cap = std::max(njobs, 1);
   w.bytes = F;
   w.generation = ++c->ws_generations;
@@ -174,14 +177,22 @@ int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
 }
 
 hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main,
-                               int fork) {
+                               int fork, const AlignArgs* split, int split_grid) {
   hipError_t e;
   if (!fork) {
+    // the split pairs are the long ones: first
+    if (split && (e = launch_align_split(*split, split_grid, main)) != hipSuccess) return e;
     for (int sh = 0; sh <= smax; ++sh)
       if ((e = launch_align(sh, args[sh], grids[sh], main)) != hipSuccess) return e;
     return hipSuccess;
   }
   if ((e = hipEventRecord(c->fork, main)) != hipSuccess) return e;
+  if (split) {
+    hipStream_t q = c->side[kSplitList];
+    if ((e = hipStreamWaitEvent(q, c->fork, 0)) != hipSuccess) return e;
+    if ((e = launch_align_split(*split, split_grid, q)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->join[kSplitList], q)) != hipSuccess) return e;
+  }
   for (int sh = 0; sh <= smax; ++sh) {
     if ((e = hipStreamWaitEvent(c->side[sh], c->fork, 0)) != hipSuccess) return e;
     if ((e = launch_align(sh, args[sh], grids[sh], c->side[sh])) != hipSuccess) return e;
@@ -189,6 +200,7 @@ hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, con
   }
   for (int sh = 0; sh <= smax; ++sh)
     if ((e = hipStreamWaitEvent(main, c->join[sh], 0)) != hipSuccess) return e;
+  if (split && (e = hipStreamWaitEvent(main, c->join[kSplitList], 0)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -226,7 +238,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   const JobDev jd = s.job(p);
   HIP_TRY(hipMemcpyAsync(w.jobs, &jd, sizeof(JobDev), hipMemcpyHostToDevice, q), "job table");
   HIP_TRY(hipMemsetAsync(w.stats, 0, kStatWords * sizeof(unsigned long long), q), "memset");
-  HIP_TRY(hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q), "memset");
+  HIP_TRY(hipMemsetAsync(w.counters, 0, kNumCounters * sizeof(int32_t), q), "memset");
   HIP_TRY(launch_init_jobs(w.jobs, &jd, 1, 1, nullptr, q), "init");
   int64_t launches = 1;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -276,7 +288,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       aa[sh].jobs = w.jobs;
       aa[sh].items = w.items + (int64_t)sh * w.item_cap;
       aa[sh].count = w.counters + sh;
-      aa[sh].work = w.counters + kNumShapes + sh;
+      aa[sh].work = w.counters + kWorkCounter0 + sh;
       aa[sh].result = nullptr;
       aa[sh].stats = w.stats;
       aa[sh].skip_absorbed = 0;
@@ -293,7 +305,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
     ra.jobs = w.jobs;
     ra.njobs = 1;
     ra.counters = w.counters;
-    ra.ncounters = 2 * kNumShapes;
+    ra.ncounters = kNumCounters;
     ra.jobs_left = nullptr;
     ra.stats = w.stats;
     ra.pair_budget = w.item_cap;
@@ -361,7 +373,10 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     if (cells_exec_out) *cells_exec_out = 0;
     return GSDR_OK;
   }
-  const int min_shape = shape < 0 ? 0 : std::min(shape, kNumShapes - 1);
+  // shape 12 + s: split pairs starting from 64-lane shape s (4..11)
+  const bool split = shape >= kNumShapes;
+  const int min_shape = shape < 0 ? 0 : split ? std::min(std::max(shape - kNumShapes, 4), kNumShapes - 1)
+                                              : std::min(shape, kNumShapes - 1);
   // a private store: sequences, plus one query slot per pair (nseq + pair)
   // that points at the query's packed words and carries the pair's cutoff
   const int64_t nslots = (int64_t)nseq + npairs;
@@ -374,8 +389,8 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     m[nseq + i] = p.m[qq];
     bcv[nseq + i] = bc[i];
   }
-  std::vector<std::vector<AlignItem>> bins(kNumShapes);
-  std::vector<std::vector<int>> bin_pair(kNumShapes);
+  std::vector<std::vector<AlignItem>> bins(kNumShapes + 1);  // the shape lists, then the split list
+  std::vector<std::vector<int>> bin_pair(kNumShapes + 1);
   for (int i = 0; i < npairs; ++i) {
     const int t = pairs[2 * i];
     const int cslot = nseq + i;  // the query, carrying the pair's cutoff
@@ -387,8 +402,13 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     it.query = sw ? t : cslot;
     it.job = 0;
     it.slot = sw ? kSwapFlag : 0;
-    bins[sh].push_back(it);
-    bin_pair[sh].push_back(i);
+    int list = sh;
+    if (split && (sw ? p.m[t] : m[cslot]) >= 64) {
+      it.slot |= sh << kSplitShapeShift;
+      list = kSplitList;
+    }
+    bins[list].push_back(it);
+    bin_pair[list].push_back(i);
   }
   std::vector<void*> A;
   int64_t F = 0;
@@ -403,7 +423,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
   if ((st = dev_alloc(A, F, &d_pk, p.pk.size() / 2)) || (st = dev_alloc(A, F, &d_pkoff, (size_t)nslots + 1)) ||
       (st = dev_alloc(A, F, &d_m, (size_t)nslots)) || (st = dev_alloc(A, F, &d_bc, (size_t)nslots)) ||
       (st = dev_alloc(A, F, &items, (size_t)npairs)) || (st = dev_alloc(A, F, &res, (size_t)npairs)) ||
-      (st = dev_alloc(A, F, &ctr, (size_t)2 * kNumShapes)) || (st = dev_alloc(A, F, &d_job, (size_t)1)) ||
+      (st = dev_alloc(A, F, &ctr, (size_t)kNumCounters)) || (st = dev_alloc(A, F, &d_job, (size_t)1)) ||
       (st = dev_alloc(A, F, &d_stats, (size_t)kStatWords))) {
     release();
     return st;
@@ -415,9 +435,9 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
   jd.st.m = d_m;
   jd.st.bc = d_bc;
   std::vector<AlignItem> all;
-  std::vector<int32_t> counts(2 * kNumShapes, 0);
-  std::vector<int> first(kNumShapes, 0);
-  for (int sh = 0; sh < kNumShapes; ++sh) {
+  std::vector<int32_t> counts(kNumCounters, 0);
+  std::vector<int> first(kNumShapes + 1, 0);
+  for (int sh = 0; sh <= kNumShapes; ++sh) {
     first[sh] = (int)all.size();
     counts[sh] = (int)bins[sh].size();
     all.insert(all.end(), bins[sh].begin(), bins[sh].end());
@@ -440,7 +460,7 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     aa.jobs = d_job;
     aa.items = items + first[sh];
     aa.count = ctr + sh;
-    aa.work = ctr + kNumShapes + sh;
+    aa.work = ctr + kWorkCounter0 + sh;
     aa.result = res + first[sh];
     aa.stats = d_stats;
     aa.skip_absorbed = 0;
@@ -449,13 +469,25 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     const int grid = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_per_cu(sh)));
     if ((e = launch_align(sh, aa, grid, q)) != hipSuccess) return bail(e, "align launch");
   }
+  if (counts[kSplitList] > 0) {
+    AlignArgs aa;
+    aa.jobs = d_job;
+    aa.items = items + first[kSplitList];
+    aa.count = ctr + kSplitList;
+    aa.work = ctr + kWorkCounter0 + kSplitList;
+    aa.result = res + first[kSplitList];
+    aa.stats = d_stats;
+    aa.skip_absorbed = 0;
+    const int grid = std::max(1, std::min(counts[kSplitList], c->cu_count * 2));
+    if ((e = launch_align_split(aa, grid, q)) != hipSuccess) return bail(e, "split launch");
+  }
   std::vector<int32_t> got(all.size());
   unsigned long long hall[kStatWords], hst[ST_NUM];
   if ((e = hipMemcpyAsync(got.data(), res, got.size() * 4, hipMemcpyDeviceToHost, q)) ||
       (e = hipMemcpyAsync(hall, d_stats, sizeof(hall), hipMemcpyDeviceToHost, q)) || (e = hipStreamSynchronize(q)))
     return bail(e, "align_pairs download");
   stats_fold(hall, hst);
-  for (int sh = 0; sh < kNumShapes; ++sh)
+  for (int sh = 0; sh <= kNumShapes; ++sh)
     for (size_t j = 0; j < bin_pair[sh].size(); ++j) accept_out[bin_pair[sh][j]] = got[first[sh] + j];
   if (cells_exec_out) *cells_exec_out = (double)hst[ST_CELLS_EXEC];
   release();

@@ -20,6 +20,7 @@ struct gsdr_records {
 struct Workspace {
   gsdr::AlignItem* items = nullptr;
   int64_t item_cap = 0;            // items per shape
+  int64_t split_cap = 0;           // items of the split list (after the shape lists)
   int32_t* counters = nullptr;     // [kNumShapes] bin counts, [kNumShapes] work counters
   unsigned long long* stats = nullptr;
   gsdr::JobDev* jobs = nullptr;    // device job table
@@ -42,6 +43,7 @@ struct Tuning {
   int screen = 1;        // word-signature screen before the phase-B filter (k >= 6)
   gsdr::ShapeTable shapes = {};  // tile shapes per text-length bin (zero: the narrowest that holds the band)
   int win_period = 0;    // 64-lane tiles: blocks between DP-edge window rebases (0 = by width)
+  int split_rows = 0;    // phase-A pairs with this many rows run on two waves (0 = never)
 };
 
 struct gsdr_ctx {
@@ -60,9 +62,9 @@ struct gsdr_ctx {
   Workspace ws;
   // side streams (default priority) that run the tile-shape kernels of a
   // step concurrently, with fork/join events
-  hipStream_t side[gsdr::kNumShapes] = {};
+  hipStream_t side[gsdr::kNumShapes + 1] = {};  // the last one runs the split list
   hipEvent_t fork = nullptr;
-  hipEvent_t join[gsdr::kNumShapes] = {};
+  hipEvent_t join[gsdr::kNumShapes + 1] = {};
   // device address of this device's no-allocation scratch array
   // (gsdr_cluster_noalloc), looked up when the context opens
   void* scratch = nullptr;
@@ -147,8 +149,9 @@ struct AlignArgs;
 // Launches the alignment kernels of shapes 0..smax. fork = 1: concurrently
 // on the side streams, forked from and joined back into `main`; 0: one after
 // another on `main` (no cross-queue waits).
+// split (optional): the split list's arguments, run beside the shapes.
 hipError_t launch_align_shapes(gsdr_ctx* c, const AlignArgs* args, int smax, const int* grids, hipStream_t main,
-                               int fork = 1);
+                               int fork = 1, const AlignArgs* split = nullptr, int split_grid = 0);
 
 int cluster_store(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out, gsdr_stats* stats);
 // fixed != nullptr: run in that workspace (the no-allocation entry's
@@ -179,6 +182,9 @@ struct StepLaunch {
   ResolveArgs ra;
   AlignArgs aa_reps[kNumShapes];
   AlignArgs aa[kNumShapes];
+  AlignArgs aa_split;  // phase A's split list
+  int split = 0;       // launch the split kernel in phase A
+  int split_grid = 1;
   int smax = 0;
   unsigned kmask = 0;  // word lengths among the jobs (bit k)
   int fork = 1;  // shape kernels on side streams
@@ -214,6 +220,7 @@ inline void tuning_shapes_out(const Tuning& T, gsdr_tuning* out) {
     out->shape_bias[b] = T.shapes.bias[b];
   }
   out->win_period = T.win_period;
+  out->split_rows = T.split_rows;
 }
 int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int max_trials, gsdr_tuning* out);
 hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev);

@@ -509,6 +509,11 @@ int align_blocks_per_cu(int shape) {
   return shape >= 0 && shape < kNumShapes ? occ[shape] : 1;
 }
 
+hipError_t launch_align_split(const AlignArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(align_split_kernel, dim3(grid), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s) {
   switch (shape) {
     case 0: return launch_align_sw<8, 1>(a, grid, s);

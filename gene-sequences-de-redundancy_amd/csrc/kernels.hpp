@@ -145,6 +145,21 @@ __host__ __device__ inline uint32_t sig_bucket(uint32_t w, int k) {
 
 // Speculative representatives per step are bounded by the accept-mask width.
 constexpr int kMaxSpec = 64;
+
+// Split list: phase-A pairs (speculative representatives against each
+// other) with many rows, each aligned by two waves of one block, the first
+// half of the rows forward and the second half backward on the reversed
+// pair, joined at the middle row (align_split_kernel). It sits after the
+// kNumShapes shape lists; a job holds at most kMaxSpec (kMaxSpec - 1) / 2
+// phase-A pairs, so it never overflows. An item's slot field carries the
+// tile shape its band picked (the widest window the halves start from).
+constexpr int kSplitList = kNumShapes;
+constexpr int kSplitPerJob = kMaxSpec * (kMaxSpec - 1) / 2;
+constexpr int kSplitShapeShift = 12;
+// Workspace counters: the item count of every list (shape lists, then the
+// split list), then the work counter of every list.
+constexpr int kWorkCounter0 = kNumShapes + 1;
+constexpr int kNumCounters = 2 * (kNumShapes + 1);
 // Candidate positions per resolve tile (256 threads, 8 each).
 constexpr int kResolveTile = 2048;
 // Candidates per filter work unit (one per 16-lane group of a 256-thread block).

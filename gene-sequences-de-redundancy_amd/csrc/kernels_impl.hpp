@@ -19,6 +19,10 @@ struct FilterArgs {
   unsigned long long* stats;
   int32_t screen;           // phase B: skip what the signature screen rejected
   ShapeTable shapes = {};   // tile-shape choice per text-length bin (zero: narrowest of all)
+  int32_t split_rows = 0;   // phase A: pairs of 64-lane tiles with this many rows or more go to the
+                            // split list (0 = none)
+  int64_t split_cap = 0;    // pairs the split list takes (it follows the kNumShapes shape lists);
+                            // later ones go to their shape lists
 };
 
 struct ScreenArgs {
@@ -44,8 +48,8 @@ struct AlignArgs {
   int32_t* result;          // non-null: result[item] = 1 accept / 0 reject (pair-level entry)
   unsigned long long* stats;
   int32_t skip_absorbed;    // skip (slot, q) when an earlier true representative already accepted q
-  int32_t win_period = 0;   // 64-lane tiles: 32-row blocks between DP-edge window rebases (0 = 2 for
-                            // 6 or more words per lane, 4 for fewer)
+  int32_t win_period = 0;   // 64-lane tiles: 32-row blocks between DP-edge window rebases, a power of
+                            // two (0 = 2 for 6 or more words per lane, 4 for fewer)
 };
 
 struct ResolveArgs {
@@ -455,10 +459,19 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         it.query = sw ? r : q;
         it.job = job;
         it.slot = slot | (sw ? kSwapFlag : 0);
+        // long phase-A pairs of the 64-lane tiles: two waves each (the
+        // split list), starting from the window of their shape
+        if (a.phase == PHASE_REPS && a.split_rows > 0 && sh >= 4 && (sw ? n : mq) >= a.split_rows) {
+          it.slot |= sh << kSplitShapeShift;
+          sh = kSplitList;
+        }
       }
+      const int sh_own = sh == kSplitList ? (it.slot >> kSplitShapeShift) & 15 : sh;
       const int wl = tid & 63;
       bool todo = pass;
-      for (uint64_t pend = __builtin_amdgcn_ballot_w64(todo); pend != 0;) {
+      while (true) {
+        const uint64_t pend = __builtin_amdgcn_ballot_w64(todo);
+        if (pend == 0) break;
         const int l = (int)__builtin_ctzll(pend);
         const int s0 = __shfl(sh, l);
         const uint64_t same = __builtin_amdgcn_ballot_w64(todo && sh == s0);
@@ -467,17 +480,23 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         base = __shfl(base, l);
         if (todo && sh == s0) {
           const int at = base + (int)__popcll(same & ((1ull << wl) - 1ull));
-          if (at < a.item_cap) {
+          if (at < (sh == kSplitList ? a.split_cap : a.item_cap)) {
             a.items[(int64_t)sh * a.item_cap + at] = it;
+            todo = false;
+          } else if (sh == kSplitList) {
+            // the split list holds as many pairs as the split kernel has
+            // blocks: the rest go to their own shape list (next round)
+            atomicSub(&a.bin_count[sh], 1);
+            it.slot &= ~(15 << kSplitShapeShift);
+            sh = sh_own;
           } else {
             // the list is full: count it so the host fails the call (the
             // host bounds k_spec * nc so that this does not happen)
             atomicAdd(&a.stats[ST_OVERFLOW], 1ull);
             atomicSub(&a.bin_count[sh], 1);
+            todo = false;
           }
-          todo = false;
         }
-        pend &= ~same;
       }
     }
   }
@@ -1296,8 +1315,33 @@ struct Pair64 {
   const uint2* pk;
   int64_t toff, qoff;  // first packed word of the bit-vector (text) and row (query) sequences
   int n, mq, c, ntw;   // text length, row count, cutoff + 1, text words
-  int every;           // blocks between DP-edge window rebases (0 = by width)
+  int every;           // blocks between DP-edge window rebases, minus one (a power of two; -1 = by width)
+  // A half of a split pair (align_split_kernel) runs only its first `rows`
+  // rows (the bounds still count all mq rows), on the reversed pair when
+  // `rev` is set, and stops early when the other half has decided (*stop).
+  int rows;
+  int rev;
+  volatile int* stop;
 };
+
+// Text word g of a sequence read backwards (bases len-1, len-2, ... from
+// bit 0 of word 0), both planes: a 32-base window of the forward packing,
+// bit-reversed; the reversed sequence's padding bits are zero like the
+// forward one's.
+__device__ __forceinline__ uint2 rev_word(const uint2* pk, int64_t off, int len, int g) {
+  const int p = len - 32 * (g + 1);
+  if (p <= -32) return make_uint2(0u, 0u);
+  const uint2 w = window32(pk, off, p > 0 ? p : 0, p > 0 ? 32 : 32 + p);
+  const int sh = p > 0 ? 0 : -p;
+  return make_uint2(__builtin_bitreverse32(w.x) >> sh, __builtin_bitreverse32(w.y) >> sh);
+}
+
+__device__ __forceinline__ uint2 text_word64(const Pair64& P, int g) {
+  return P.rev ? rev_word(P.pk, P.toff, P.n, g) : P.pk[P.toff + g];
+}
+__device__ __forceinline__ uint2 query_word64(const Pair64& P, int jb) {
+  return P.rev ? rev_word(P.pk, P.qoff, P.mq, jb) : P.pk[P.qoff + jb];
+}
 
 // Per-wave LDS staging used to re-lay a window.
 struct Stage64 {
@@ -1319,7 +1363,7 @@ __device__ inline int width_to(int base, int last, int wmax) {
 // Loads a window of W words per lane based at word b. Words that were in
 // the staged window (based at ob, ow words per lane) come from LDS, the rest
 // start all ones with their text from the store.
-template <int W>
+template <int W, bool SPLIT>
 __device__ __forceinline__ void load_window64(const Pair64& P, const Stage64& X, int lane, int b, int ob, int ow,
                                               uint32_t (&v)[W], uint32_t (&tl)[W], uint32_t (&th)[W]) {
   // selects rather than branches: writing the word arrays in divergent
@@ -1332,7 +1376,7 @@ __device__ __forceinline__ void load_window64(const Pair64& P, const Stage64& X,
     const int ri = staged ? rel : 0;
     const uint32_t sv = X.v[ri], sl = X.tl[ri], sh = X.th[ri];
     uint2 t = make_uint2(0u, 0u);
-    if (!staged && g < P.ntw) t = P.pk[P.toff + g];
+    if (!staged && g < P.ntw) t = SPLIT ? text_word64(P, g) : P.pk[P.toff + g];
     v[w] = staged ? sv : ~0u;
     tl[w] = staged ? sl : t.x;
     th[w] = staged ? sh : t.y;
@@ -1364,19 +1408,19 @@ struct Win64 {
 // Runs 32-row blocks of the pair with W words per lane, from the state in
 // S (window contents staged in X). Returns 1 (accept) or 0 (reject), or -1
 // after staging the window when the next block needs another width (S.w).
-template <int W>
+template <int W, bool SPLIT>
 __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, int lane, Win64& S,
                                             unsigned long long& cells, unsigned long long& cdec) {
   const int n = P.n, mq = P.mq, c = P.c;
   int b = S.b, jb = S.jb, frozen = S.frozen;
   uint32_t v[W], tl[W], th[W];
-  load_window64<W>(P, X, lane, b, S.ob, S.ow, v, tl, th);
+  load_window64<W, SPLIT>(P, X, lane, b, S.ob, S.ow, v, tl, th);
   while (true) {
     // ---- the 32 rows of block jb (fewer in the last block)
-    const uint2 qw = P.pk[P.qoff + jb];
+    const uint2 qw = SPLIT ? query_word64(P, jb) : P.pk[P.qoff + jb];
     const uint32_t qls = __builtin_amdgcn_readfirstlane(qw.x);
     const uint32_t qhs = __builtin_amdgcn_readfirstlane(qw.y);
-    const int nrows = min(32, mq - jb * 32);
+    const int nrows = min(32, (SPLIT ? P.rows : mq) - jb * 32);
     if constexpr (W >= 6) {
       // With 6 or more words per lane each row branches (scalar) to the
       // code specialised for its base; the empty asm keeps the four match
@@ -1452,6 +1496,16 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
       if (lane == 0) cdec += (unsigned long long)band_cells_rows(n, mq, c, jl + 1);
       return z >= c ? 1 : 0;
     }
+    if constexpr (SPLIT) {
+      if (jl + 1 >= P.rows) {
+        // a half of a split pair at its last row, undecided: its window goes
+        // to LDS for the join at the middle row
+        stage_window64<W>(X, lane, v, tl, th);
+        S.b = b; S.jb = jb + 1; S.frozen = frozen; S.ob = b; S.ow = W; S.w = W;
+        return -2;
+      }
+      if (*P.stop) return -3;  // the other half has decided
+    }
     ++jb;
 
     // ---- window for the next block: from the first live word, reaching
@@ -1464,8 +1518,11 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     const int last = hi_col >> 5;
     int lo_col = max(0, jn - (mq - c));
     const bool covered = b + 64 * W > last;
-    const int every = P.every > 0 ? P.every : (W >= 6 ? 2 : 4);
-    if (!covered || (W > 2 && jb % every == 0)) {
+    // periods are powers of two: a mask (P.every = period - 1, or -1 for
+    // the default of this width)
+    constexpr int kEvery = W >= 6 ? 2 : 4;
+    const bool rebase = (jb & (P.every >= 0 ? P.every : kEvery - 1)) == 0;
+    if (!covered || (W > 2 && rebase)) {
       // inclusive prefix over the lanes: D at the end of each lane's words;
       // lanes whose last column has D + R < c are dead (a prefix of lanes)
       uint32_t pre = x & 0xffffu;
@@ -1501,31 +1558,33 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
           S.b = nb; S.jb = jb; S.frozen = frozen; S.ob = b; S.ow = W; S.w = w2;
           return -1;
         }
-        load_window64<W>(P, X, lane, nb, b, W, v, tl, th);
+        load_window64<W, SPLIT>(P, X, lane, nb, b, W, v, tl, th);
         b = nb;
       }
     }
   }
 }
 
-// Decides LCS >= c for a pair on a 64-lane tile of at most W words per lane.
-template <int W>
-__device__ __forceinline__ int decide64(const Pair64& P, const Stage64& X, int lane, unsigned long long& cells,
-                                        unsigned long long& cdec) {
-  Win64 S{0, 0, 0, 0, 0, W, W};
+// Decides LCS >= c for a pair on a 64-lane tile of at most W words per lane,
+// from the window state S (width S.w, at most S.w0 <= W). Returns 1 / 0, or
+// for a half of a split pair -2 (rows done, window staged in X) or -3
+// (stopped: the other half decided).
+template <int W, bool SPLIT>
+__device__ __forceinline__ int decide64(const Pair64& P, const Stage64& X, int lane, Win64& S,
+                                        unsigned long long& cells, unsigned long long& cdec) {
   while (true) {
     int r = -1;
     switch (S.w) {
-      case 16: if constexpr (W >= 16) r = run_blocks64<16>(P, X, lane, S, cells, cdec); break;
-      case 12: if constexpr (W >= 12) r = run_blocks64<12>(P, X, lane, S, cells, cdec); break;
-      case 10: if constexpr (W >= 10) r = run_blocks64<10>(P, X, lane, S, cells, cdec); break;
-      case 8: if constexpr (W >= 8) r = run_blocks64<8>(P, X, lane, S, cells, cdec); break;
-      case 6: if constexpr (W >= 6) r = run_blocks64<6>(P, X, lane, S, cells, cdec); break;
-      case 4: if constexpr (W >= 4) r = run_blocks64<4>(P, X, lane, S, cells, cdec); break;
-      case 3: if constexpr (W >= 3) r = run_blocks64<3>(P, X, lane, S, cells, cdec); break;
-      default: r = run_blocks64<2>(P, X, lane, S, cells, cdec); break;
+      case 16: if constexpr (W >= 16) r = run_blocks64<16, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 12: if constexpr (W >= 12) r = run_blocks64<12, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 10: if constexpr (W >= 10) r = run_blocks64<10, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 8: if constexpr (W >= 8) r = run_blocks64<8, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 6: if constexpr (W >= 6) r = run_blocks64<6, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 4: if constexpr (W >= 4) r = run_blocks64<4, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 3: if constexpr (W >= 3) r = run_blocks64<3, SPLIT>(P, X, lane, S, cells, cdec); break;
+      default: r = run_blocks64<2, SPLIT>(P, X, lane, S, cells, cdec); break;
     }
-    if (r >= 0) return r;
+    if (r != -1) return r;
   }
 }
 
@@ -1535,7 +1594,8 @@ __device__ inline void align_loop64(const AlignArgs& a) {
   const int wid = threadIdx.x >> 6;
   __shared__ uint32_t stage[4][3][64 * W];
   const Stage64 X{stage[wid][0], stage[wid][1], stage[wid][2]};
-  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0, nskip = 0;
+  unsigned long long cells = 0, cdec = 0;
+  uint32_t nacc = 0, nrej = 0, nskip = 0;  // pairs per wave (32-bit: the u64 forms spilled in 64x8)
   const int total = *a.count;
   if (total == 0 || work_taken(a.work, total)) return;
   while (true) {
@@ -1559,13 +1619,15 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     }
     int res = 0;
     if (c <= min(n, mq)) {
-      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1, a.win_period};
+      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1,
+                     a.win_period - 1, mq, 0, nullptr};
       // a step ends when its longest pair does: waves on long pairs get the
       // SIMD's issue slots first, the short pairs fill the gaps
       if (mq >= 16384) __builtin_amdgcn_s_setprio(3);
       else if (mq >= 4096) __builtin_amdgcn_s_setprio(2);
       else if (mq >= 1024) __builtin_amdgcn_s_setprio(1);
-      res = decide64<W>(P, X, lane, cells, cdec);
+      Win64 S{0, 0, 0, 0, 0, W, W};
+      res = decide64<W, false>(P, X, lane, S, cells, cdec);
       __builtin_amdgcn_s_setprio(0);
     }
     if (lane == 0) {
@@ -1581,7 +1643,196 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     unsigned long long* sr = stat_row(a.stats);
     if (cells) atomicAdd(&sr[ST_CELLS_EXEC], cells);
     if (cdec) atomicAdd(&sr[ST_CELLS_DECIDED], cdec);
-    if (nskip) atomicAdd(&sr[ST_SPEC_WASTE], nskip);
+    if (nskip) atomicAdd(&sr[ST_SPEC_WASTE], (unsigned long long)nskip);
+    if (nacc) atomicAdd(&sr[ST_ACCEPT], (unsigned long long)nacc);
+    if (nrej) atomicAdd(&sr[ST_REJ_ALIGN], (unsigned long long)nrej);
+  }
+}
+
+// ----------------------------------------------------------- split pairs
+//
+// A long pair of the step's speculative representatives against each other
+// ends its phase when it runs alone: one wave's chain of dependent rows. Its
+// rows split at h (a multiple of 32 near m/2): wave 0 of a 128-thread block
+// runs rows 0..h-1 of (text, query), wave 1 the other m-h rows on the
+// reversed pair (reversing both sequences keeps every common subsequence),
+// each with the banded window, exact early decisions and rebases of the
+// 64-lane tile code, the bounds counting the rows of both halves (R = rows
+// left of all m). A half that decides alone decides the pair: its accept is
+// a common subsequence of c bases, its reject bounds every path through all
+// m rows. Otherwise the halves join at the middle row (Hirschberg's split):
+// with F(i) the forward value over text columns < i and B(i) the backward
+// value over columns >= i, LCS >= F(i) + B(i) for every i. For a path of at
+// least c matches, let A be one past its last forward match and Bc its
+// first backward match column: no column in [A, Bc] is ever frozen by
+// either half (each such column keeps the path's count plus what it can
+// still gain at c or more), the window values there are at least the
+// path's forward and backward match counts (monotone in the column), so
+// some i in [A, Bc] inside both windows has F(i) + B(i) >= c. Hence the pair
+// is accepted iff the maximum over the columns both windows hold reaches c.
+
+// Zero flags (columns with a match recorded) of a staged window, as text
+// word g of the staged sequence's own orientation; 0 outside the window and
+// in padding columns.
+__device__ __forceinline__ uint32_t staged_zeros(const uint32_t* v, int base, int width, int n, int g) {
+  const int k = g - base;
+  if (k < 0 || k >= 64 * width) return 0u;
+  return ~v[k] & valid_bits(g, n);
+}
+
+// The join at the middle row (wave 0): forward window (base bf words,
+// width wf, ff zeros frozen left of it), backward window in reversed
+// coordinates (bb, wb, fb). Columns i range over [32 bf, n - 32 bb]; F(i) =
+// ff + forward zeros in [32 bf, i), B(i) = fb + backward zeros in
+// [i, n - 32 bb). Returns 1 when max_i F(i) + B(i) >= c.
+__device__ int join_halves(const uint32_t* vf, int bf, int wf, int ff, const uint32_t* vb, int bb, int wb, int fb,
+                           int n, int c, int lane) {
+  const int L = 32 * bf, U = n - 32 * bb;
+  if (L > U) return 0;  // no column is live in both halves: no path of c matches
+  const int g0 = bf, g1 = (U + 31) >> 5;  // text words holding columns [L, U)
+  const int per = (g1 - g0 + 63) >> 6;
+  const int ga = g0 + lane * per, gz = min(g1, ga + per);
+  int run = 0, best = 0, btot = 0;
+  for (int g = ga; g < gz; ++g) {
+    uint32_t za = staged_zeros(vf, bf, wf, n, g);
+    // backward zeros of columns 32g .. 32g+31: reversed columns n-1-32g
+    // down to n-32-32g, the 32 reversed bits from p = n - 32(g+1), bit-reversed
+    const int p = n - 32 * (g + 1);
+    const int rw = p >> 5, sft = p & 31;  // floor division for negative p
+    const uint32_t lo = staged_zeros(vb, bb, wb, n, rw), hi = staged_zeros(vb, bb, wb, n, rw + 1);
+    uint32_t zb = __builtin_bitreverse32(sft ? (lo >> sft) | (hi << (32 - sft)) : lo);
+    if (g == (U >> 5)) {  // the word holding U: columns < U only
+      const uint32_t keep = (1u << (U & 31)) - 1u;
+      za &= keep;
+      zb &= keep;
+    }
+    btot += __popc(zb);
+    for (int t = 0; t < 32; ++t) {
+      run += (int)((za >> t) & 1u) - (int)((zb >> t) & 1u);
+      best = max(best, run);
+    }
+  }
+  // value(i) = ff + fb + (backward zeros of [L, U)) + sum over [L, i) of
+  // (forward - backward zeros): the best prefix over all lanes
+  int pre = run;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(pre, d);
+    if (lane >= d) pre += y;
+  }
+  int m = pre - run + best;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m = max(m, __shfl_xor(m, off));
+    btot += __shfl_xor(btot, off);
+  }
+  return ff + fb + btot + m >= c ? 1 : 0;
+}
+
+// One split pair per 128-thread block at a time, from the split list (phase
+// A: no skipping of absorbed candidates). With more pairs than blocks the
+// halves would only queue behind each other, so each wave then aligns whole
+// pairs.
+__global__ __launch_bounds__(128) void align_split_kernel(AlignArgs a) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __shared__ uint32_t stage[2][3][64 * 16];
+  __shared__ int s_item, s_stop;
+  __shared__ int s_res[2], s_b[2], s_w[2], s_frozen[2];
+  const Stage64 X{stage[wid][0], stage[wid][1], stage[wid][2]};
+  unsigned long long cells = 0, nacc = 0, nrej = 0, cdec = 0;
+  const int total = *a.count;
+  if (total == 0) return;
+  const bool whole = total > (int)gridDim.x;  // the same in every block
+  while (whole) {
+    int got = 0;
+    if (lane == 0) got = atomicAdd(a.work, 1);
+    got = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
+    if (got >= total) break;
+    const AlignItem it = a.items[got];
+    const int text = uniform(it.text), query = uniform(it.query), jobi = uniform(it.job), rslot = uniform(it.slot);
+    const JobDev& J = a.jobs[jobi];
+    const StoreView& st = J.st;
+    const int islot = rslot & (kSwapFlag - 1);
+    const int cand = (rslot & kSwapFlag) ? text : query;
+    const int n = uniform(st.m[text]), mq = uniform(st.m[query]), c = uniform(st.bc[cand]) + 1;
+    int res = 0;
+    if (!(rslot & kPigeonFlag) && c <= min(n, mq)) {
+      const int w0 = max(2, kShapes[(rslot >> kSplitShapeShift) & 15].w);
+      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1,
+                     a.win_period - 1, mq, 0, nullptr};
+      Win64 S{0, 0, 0, 0, 0, w0, w0};
+      res = decide64<16, false>(P, X, lane, S, cells, cdec);
+    }
+    if (lane == 0) {
+      if (a.result) {
+        a.result[got] = res;
+      } else if (res) {
+        atomicOr(&J.acc[cand], 1ull << islot);
+      }
+      if (res) ++nacc; else ++nrej;
+    }
+  }
+  while (!whole) {
+    __syncthreads();  // the previous pair's staged windows are no longer read
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(a.work, 1);
+      s_stop = 0;
+    }
+    __syncthreads();
+    const int got = s_item;
+    if (got >= total) break;  // the same for both waves
+    const AlignItem it = a.items[got];
+    const int text = uniform(it.text), query = uniform(it.query), jobi = uniform(it.job), rslot = uniform(it.slot);
+    const JobDev& J = a.jobs[jobi];
+    const StoreView& st = J.st;
+    const int islot = rslot & (kSwapFlag - 1);
+    const int cand = (rslot & kSwapFlag) ? text : query;
+    const int n = uniform(st.m[text]), mq = uniform(st.m[query]), c = uniform(st.bc[cand]) + 1;
+    // both halves hold at least 32 rows (the split list takes m >= 64)
+    const bool work = !(rslot & kPigeonFlag) && c <= min(n, mq) && mq >= 64;
+    if (work) {
+      const int h = ((mq >> 1) >> 5) << 5;
+      const int w0 = max(2, kShapes[(rslot >> kSplitShapeShift) & 15].w);
+      const Pair64 P{st.pk, uniform64(st.pkoff[text]), uniform64(st.pkoff[query]), n, mq, c, (n >> 5) + 1,
+                     a.win_period - 1, wid == 0 ? h : mq - h, wid, &s_stop};
+      Win64 S{0, 0, 0, 0, 0, w0, w0};
+      const int r = decide64<16, true>(P, X, lane, S, cells, cdec);
+      if (lane == 0) {
+        if (r >= 0) s_stop = 1;  // the other half may stop
+        s_res[wid] = r;
+        s_b[wid] = S.b;
+        s_w[wid] = S.ow;
+        s_frozen[wid] = S.frozen;
+      }
+    }
+    __syncthreads();
+    if (wid == 0) {
+      int res = 0;
+      if (work) {
+        const int r0 = s_res[0], r1 = s_res[1];
+        if (r0 == 1 || r1 == 1) {
+          res = 1;
+        } else if (r0 == 0 || r1 == 0) {
+          res = 0;
+        } else {
+          res = join_halves(stage[0][0], s_b[0], s_w[0], s_frozen[0], stage[1][0], s_b[1], s_w[1], s_frozen[1], n,
+                            c, lane);
+        }
+      }
+      if (lane == 0) {
+        if (a.result) {
+          a.result[got] = res;
+        } else if (res) {
+          atomicOr(&J.acc[cand], 1ull << islot);
+        }
+        if (res) ++nacc; else ++nrej;
+      }
+    }
+  }
+  if (lane == 0 && a.stats) {
+    unsigned long long* sr = stat_row(a.stats);
+    if (cells) atomicAdd(&sr[ST_CELLS_EXEC], cells);
+    if (cdec) atomicAdd(&sr[ST_CELLS_DECIDED], cdec);
     if (nacc) atomicAdd(&sr[ST_ACCEPT], nacc);
     if (nrej) atomicAdd(&sr[ST_REJ_ALIGN], nrej);
   }
@@ -1611,6 +1862,8 @@ __global__ __launch_bounds__(256, align_waves_per_eu(S, W)) void align_kernel(Al
 // class of word lengths that share a table group size.
 hipError_t launch_filter(const FilterArgs& a, unsigned kmask, int grid, hipStream_t s);
 hipError_t launch_align(int shape, const AlignArgs& a, int grid, hipStream_t s);
+// The split list (kSplitList): one pair per 128-thread block at a time.
+hipError_t launch_align_split(const AlignArgs& a, int grid, hipStream_t s);
 int align_blocks_per_cu(int shape);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t s);
 hipError_t launch_resolve_reps(const ResolveArgs& a, hipStream_t s);

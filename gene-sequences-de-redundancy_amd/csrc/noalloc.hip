@@ -24,12 +24,13 @@
 namespace gsdr {
 namespace {
 
-// Layout of the array: job table, shape counters, statistics, running-jobs
-// counter, then the kNumShapes alignment lists of equal capacity.
+// Layout of the array: job table, list counters, statistics, running-jobs
+// counter, then the kNumShapes alignment lists of equal capacity and the
+// split list (one job's phase-A pairs).
 constexpr size_t kJobsOff = 0, kCountersOff = 256, kStatsOff = 512;
 constexpr size_t kLeftOff = kStatsOff + kStatWords * sizeof(unsigned long long), kItemsOff = kLeftOff + 256;
 static_assert(sizeof(JobDev) <= kCountersOff - kJobsOff, "job table slot");
-static_assert(2 * kNumShapes * sizeof(int32_t) <= kStatsOff - kCountersOff, "counter slot");
+static_assert(kNumCounters * sizeof(int32_t) <= kStatsOff - kCountersOff, "counter slot");
 static_assert(kStatWords * sizeof(unsigned long long) <= kLeftOff - kStatsOff, "statistics slot");
 
 __device__ __align__(256) unsigned char scratch[GSDR_NOALLOC_SCRATCH_BYTES];
@@ -70,10 +71,11 @@ int cluster_noalloc(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* o
   w.stats = reinterpret_cast<unsigned long long*>(base + kStatsOff);
   w.jobs_left = reinterpret_cast<int32_t*>(base + kLeftOff);
   w.items = reinterpret_cast<AlignItem*>(base + kItemsOff);
-  w.item_cap = (int64_t)((GSDR_NOALLOC_SCRATCH_BYTES - kItemsOff) / (sizeof(AlignItem) * kNumShapes));
+  constexpr size_t kListBytes = GSDR_NOALLOC_SCRATCH_BYTES - kItemsOff - kSplitPerJob * sizeof(AlignItem);
+  w.item_cap = (int64_t)(kListBytes / (sizeof(AlignItem) * kNumShapes));
+  w.split_cap = kSplitPerJob;
   w.bytes = GSDR_NOALLOC_SCRATCH_BYTES;
-  static_assert((GSDR_NOALLOC_SCRATCH_BYTES - kItemsOff) / (sizeof(AlignItem) * kNumShapes) >=
-                    (size_t)GSDR_NOALLOC_MAX_SEQS,
+  static_assert(kListBytes / (sizeof(AlignItem) * kNumShapes) >= (size_t)GSDR_NOALLOC_MAX_SEQS,
                 "every sequence fits each shape list once");
   if (p.mode == GSDR_MODE_PLAIN) return cluster_plain(c, s, p, out, stats, &w);
   if (p.mode != GSDR_MODE_BATCHED) return fail(GSDR_E_INVALID, "unknown mode");

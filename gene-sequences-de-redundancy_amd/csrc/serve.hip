@@ -110,7 +110,7 @@ int gsdr_serve_open(gsdr_ctx* c, int64_t hbm_budget, int32_t max_slots, gsdr_ser
   int st = workspace_reserve(c, 32ll << 20, max_slots);
   if (st) return st;
   hipError_t e;
-  if ((e = hipMemsetAsync(c->ws.counters, 0, 2 * kNumShapes * sizeof(int32_t), c->stream)) != hipSuccess ||
+  if ((e = hipMemsetAsync(c->ws.counters, 0, kNumCounters * sizeof(int32_t), c->stream)) != hipSuccess ||
       (e = hipMemsetAsync(c->ws.stats, 0, kStatWords * sizeof(unsigned long long), c->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return hip_fail(e, "serve init");
@@ -235,7 +235,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
       // fail (poll reports GSDR_E_LIMIT) and release their memory, the
       // counters are cleared, and the server goes on with later jobs.
       if ((e = hipMemsetAsync(w.stats + ST_OVERFLOW, 0, sizeof(unsigned long long), q)) != hipSuccess ||
-          (e = hipMemsetAsync(w.counters, 0, 2 * kNumShapes * sizeof(int32_t), q)) != hipSuccess ||
+          (e = hipMemsetAsync(w.counters, 0, kNumCounters * sizeof(int32_t), q)) != hipSuccess ||
           (e = hipStreamSynchronize(q)) != hipSuccess)
         return hip_fail(e, "overflow reset");
       drop_graph(srv);

@@ -30,7 +30,7 @@ bool same(const Tuning& a, const Tuning& b) {
   bool eq = a.swap_ratio == b.swap_ratio && a.spec_max == b.spec_max && a.filter_blocks == b.filter_blocks &&
             a.align_blocks == b.align_blocks && a.fork_shapes == b.fork_shapes &&
             a.list_factor == b.list_factor && a.pigeon == b.pigeon && a.screen == b.screen &&
-            a.win_period == b.win_period;
+            a.win_period == b.win_period && a.split_rows == b.split_rows;
   for (int i = 0; i < kLenBins; ++i)
     eq = eq && a.shapes.mask[i] == b.shapes.mask[i] && a.shapes.bias[i] == b.shapes.bias[i];
   return eq;
@@ -95,6 +95,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
       {&Tuning::align_blocks, {2, 8, 0}},
       {&Tuning::filter_blocks, {4, 16}},
       {&Tuning::fork_shapes, {0}},
+      {&Tuning::split_rows, {0, 1024}},
   };
   for (const Knob& k : knobs) {
     for (int v : k.values) {

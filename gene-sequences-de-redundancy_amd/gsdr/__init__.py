@@ -84,6 +84,7 @@ class Tuning(C.Structure):
         ("shape_mask", C.c_int32 * 4),
         ("shape_bias", C.c_int32 * 4),
         ("win_period", C.c_int32),
+        ("split_rows", C.c_int32),
     ]
 
     def __init__(self, **kw):

@@ -111,7 +111,11 @@ typedef struct gsdr_tuning {
   int32_t shape_mask[4];
   int32_t shape_bias[4];
   int32_t win_period;     /* 64-lane tiles: 32-row blocks between window rebases at the DP edge
-                             (0 = 2 with 6 or more words per lane, 4 with fewer; else 1..16) */
+                             (0 = 2 with 6 or more words per lane, 4 with fewer; else 1, 2, 4, 8 or 16) */
+  int32_t split_rows;     /* batched: a pair of the speculative representatives against each other
+                             on a 64-lane tile with at least this many rows is aligned by two waves,
+                             half of the rows forward and half backward, joined at the middle row
+                             (default 0 = never; else 64..32768) */
 } gsdr_tuning;
 
 /* ---- options (host side): replaces checkOption (func.cpp:6-57) and the
@@ -242,7 +246,10 @@ int gsdr_get_tuning(const gsdr_ctx *ctx, gsdr_tuning *out);
  *      LCS(text, query) > bc on the device, the test of kernel_dynamic
  *      (func.cpp:572-676). Sequences are given as raw bytes like FASTA data;
  *      pairs holds npairs (text index, query index) pairs. shape < 0 picks the
- *      tile shape per pair; otherwise it forces shape index `shape`. ---- */
+ *      tile shape per pair; 0..11 forces tile shape index `shape` (or the
+ *      narrowest wider one that holds the pair); 12 + s (s = 4..11) runs every
+ *      pair with at least 64 query bases as a split pair (two waves, joined at
+ *      the middle row) starting from shape s or wider. ---- */
 int gsdr_align_pairs(gsdr_ctx *ctx, const char *data, const int64_t *data_off, int32_t nseq,
                      const int32_t *pairs, const int32_t *bc, int32_t npairs, int32_t shape,
                      int32_t *accept_out, double *cells_exec_out);

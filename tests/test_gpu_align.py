@@ -180,3 +180,74 @@ def test_empty_and_degenerate(engine):
         bcs = np.array([max(0, L + off) for L in expect_l], dtype=np.int32)
         got, _ = engine.align_pairs(seqs, pairs, bcs)
         assert list(got) == [L > b for L, b in zip(expect_l, bcs)]
+
+
+# split pairs (gsdr_align_pairs shape 12 + s): two waves, the first half of
+# the rows forward and the second half backward on the reversed pair, joined
+# at the middle row; pairs with fewer than 64 query bases run unsplit
+
+
+@pytest.mark.parametrize("shape", [4, 5, 6, 7, 9, 11])
+def test_split_boundary_decisions(engine, shape):
+    seqs, pairs = make_pairs(200 + shape, 80, 5000)
+    ep, bcs, expect = boundary_cases(seqs, pairs)
+    got, _ = engine.align_pairs(seqs, ep, bcs, shape=12 + shape)
+    bad = np.nonzero(got != expect)[0]
+    assert len(bad) == 0, f"{len(bad)} mismatches, first: pair {ep[bad[0]]} bc {bcs[bad[0]]} expect {expect[bad[0]]}"
+
+
+@pytest.mark.parametrize("shape", [4, 8])
+def test_split_single_base_runs(engine, shape):
+    """Single-base-run texts through both halves (carries ripple through every
+    word on the reversed pair too), queries of every length class."""
+    rng = np.random.default_rng(90 + shape)
+    seqs, pairs = [], []
+    for i in range(12):
+        runs, n = [], int(rng.integers(200, 6000))
+        while sum(len(r) for r in runs) < n:
+            runs.append(bytes([int(rng.choice(ACGT))]) * int(rng.integers(33, 260)))
+        text = b"".join(runs)[:n]
+        if i % 4 == 0:
+            q = rand_seq(rng, int(rng.integers(64, max(65, n // 2))))
+        elif i % 4 == 1:
+            q = mutate(rng, text, 0.05)
+        elif i % 4 == 2:
+            q = text[: n // 3] + rand_seq(rng, 40) + text[n // 2:]
+        else:
+            q = bytes(reversed(text))
+        seqs += [text, q]
+        pairs.append((2 * i, 2 * i + 1))
+    ep, bcs, expect = boundary_cases(seqs, pairs)
+    got, _ = engine.align_pairs(seqs, ep, bcs, shape=12 + shape)
+    assert np.array_equal(got, expect)
+
+
+def test_split_near_threshold_long(engine):
+    """Split pairs up to the 32,767-base maximum, at the LCS boundary and at
+    the t=0.95 / t=0.90 cutoffs, with query lengths around 32-row blocks."""
+    rng = np.random.default_rng(17)
+    seqs, ep, bcs, expect = [], [], [], []
+    for n in (32767, 30000, 12345, 4096, 97, 64):
+        text = rand_seq(rng, n)
+        for d in (0.0, 0.02, 0.06, 0.12):
+            q = mutate(rng, text, d) or b"A"
+            L = O.lcs(text, q)
+            seqs += [text, q]
+            i = len(seqs) - 2
+            for bc in (L - 1, L, int(np.ceil(np.float32(len(q)) * np.float32(0.95))),
+                       int(np.ceil(np.float32(len(q)) * np.float32(0.9)))):
+                if bc >= 0:
+                    ep.append((i, i + 1))
+                    bcs.append(bc)
+                    expect.append(L > bc)
+    got, _ = engine.align_pairs(seqs, ep, np.array(bcs, dtype=np.int32), shape=12 + 4)
+    assert np.array_equal(got, np.array(expect))
+
+
+def test_split_list_longer_than_the_grid(engine):
+    """More split pairs than blocks: each wave aligns whole pairs instead."""
+    seqs, pairs = make_pairs(300, 240, 2000)
+    ep, bcs, expect = boundary_cases(seqs, pairs)
+    assert len(ep) > 2 * 256 + 100
+    got, _ = engine.align_pairs(seqs, ep, bcs, shape=12 + 4)
+    assert np.array_equal(got, expect)

@@ -99,7 +99,8 @@ def test_autotune_then_tuned_calls_match(engine, tmp_path):
 def test_tuning_out_of_range_is_rejected(engine):
     for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0),
                 dict(align_blocks=65), dict(align_blocks=-1), dict(fork_shapes=2), dict(list_factor=0),
-                dict(win_period=17), dict(win_period=-1), dict(shape_mask=4096), dict(shape_bias=4)]:
+                dict(win_period=17), dict(win_period=-1), dict(win_period=3), dict(shape_mask=4096), dict(shape_bias=4),
+                dict(split_rows=63), dict(split_rows=32769), dict(split_rows=-1)]:
         tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1, list_factor=64)
         for k, v in bad.items():
             if k in ("shape_mask", "shape_bias"):
@@ -110,3 +111,27 @@ def test_tuning_out_of_range_is_rejected(engine):
             engine.set_tuning(tu)
     d = engine.tuning()
     assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, 8, 4, 1, 64)
+
+
+@pytest.mark.parametrize("law,n,t", [(4, 5000, 0.95), (4, 4000, 0.80), (2, 3000, 0.99), (1, 2000, 0.90)])
+def test_split_pairs_give_the_oracle_clusters(engine, tmp_path, law, n, t):
+    """Phase-A pairs aligned by two waves (forward and backward halves joined
+    at the middle row) from 64 rows up, and never: the oracle's clusters."""
+    r, ocl = _oracle(tmp_path, gsdr.Records.generate(law, 1200 + law, n), t)
+    store = engine.load(r, gsdr.auto_word_length(t))
+    try:
+        for rows in (64, 300, 0):
+            engine.set_tuning(gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1,
+                                          list_factor=64, split_rows=rows))
+            assert engine.tuning().split_rows == rows
+            cl, st = store.cluster(t, store.k, mode=gsdr.MODE_BATCHED, tune=True)
+            assert np.array_equal(cl, ocl), rows
+            assert st.pairs == st.rej_length + st.rej_comp + st.rej_word + st.aligned
+        # concurrent and serial launch of the split kernel beside the shapes
+        engine.set_tuning(gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=0,
+                                      list_factor=64, split_rows=64))
+        cl, _ = store.cluster(t, store.k, mode=gsdr.MODE_BATCHED, tune=True)
+        assert np.array_equal(cl, ocl)
+    finally:
+        engine.set_tuning(None)
+        store.free()

@@ -40,3 +40,20 @@ def test_asm_carries_are_scalar(tmp_path):
         if t[0] == "v_addc_co_u32_e64":
             assert SCALAR.match(ops[-1]), f"carry in not in scalar registers: {line.strip()}"
     assert n > 0, "no add-with-carry instructions found"
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_no_kernel_spills_to_scratch(tmp_path):
+    """Every kernel of the per-round pipeline keeps its values in registers:
+    the compiler's resource report shows no scratch memory for any of them
+    (a spill in the alignment rows costs a memory round trip per row)."""
+    cmd = [HIPCC, "-O3", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "--cuda-device-only", "-c", "-o", str(tmp_path / "k.o"),
+           "-Rpass-analysis=kernel-resource-usage", os.path.join(PKG, "csrc", "kernels.hip")]
+    r = subprocess.run(cmd, cwd=PKG, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    names = re.findall(r"remark: Function Name: (\S+)", r.stderr)
+    scratch = [int(x) for x in re.findall(r"remark:\s+ScratchSize \[bytes/lane\]: (\d+)", r.stderr)]
+    assert len(names) == len(scratch) and len(names) >= 20, (len(names), len(scratch))
+    spilled = [n for n, s in zip(names, scratch) if s]
+    assert not spilled, spilled
