@@ -87,7 +87,7 @@ def cpu_baseline(cfg_name, threads):
     import oracle_ffi as O
 
     gen, seed, n, t, _ = CONFIGS[cfg_name]
-    sample_n = {"c1": 10000, "c2": 20000, "c4": 150000}[cfg_name]
+    sample_n = {"c1": 10000, "c2": 20000, "c4": 150000, "t99": 20000}[cfg_name]
     recs = gsdr.Records.generate(gen, seed, sample_n)
     with tempfile.TemporaryDirectory() as d:
         fa = os.path.join(d, "sample.fasta")

@@ -19,6 +19,15 @@
 
 namespace gsdr {
 
+namespace {
+// memcpy that accepts empty ranges of empty containers (whose data() may be
+// null, which memcpy itself does not allow even for zero bytes)
+inline void copy_bytes(void* dst, const void* src, size_t n) {
+  if (n) std::memcpy(dst, src, n);
+}
+}  // namespace
+
+
 // ------------------------------------------------------------------ memory
 static constexpr size_t kHuge = size_t(2) << 20;
 
@@ -105,7 +114,7 @@ int64_t copy_lines(const char* p, const char* e, char* dst) {
   while (p < e) {
     const char* nl = (const char*)memchr(p, '\n', (size_t)(e - p));
     const char* q = nl ? nl : e;
-    std::memcpy(d, p, (size_t)(q - p));
+    copy_bytes(d, p, (size_t)(q - p));
     d += q - p;
     p = q + 1;
   }
@@ -250,7 +259,7 @@ int read_fasta(const char* path, Records& out, std::string& err, int threads, bo
     for (int64_t r = lo; r < hi; ++r) {
       const int64_t i = length_order ? slot[(size_t)r] : r;
       const int64_t s = start[(size_t)r], e = start[(size_t)r + 1], h = hdr[(size_t)r];
-      std::memcpy(out.names.data() + out.name_off[(size_t)i], buf + s, (size_t)(h - s));
+      copy_bytes(out.names.data() + out.name_off[(size_t)i], buf + s, (size_t)(h - s));
       if (h + 1 < e) copy_lines(buf + h + 1, buf + e, out.data.data() + out.data_off[(size_t)i]);
     }
   });
@@ -282,9 +291,9 @@ Records permute(const Records& r, const std::vector<int64_t>& perm, int threads)
       const int64_t hi = std::min(n, lo + kChunk);
       for (int64_t i = lo; i < hi; ++i) {
         const int64_t src = perm[(size_t)i];
-        std::memcpy(o.names.data() + o.name_off[(size_t)i], r.names.data() + r.name_off[src],
+        copy_bytes(o.names.data() + o.name_off[(size_t)i], r.names.data() + r.name_off[src],
                     (size_t)(r.name_off[src + 1] - r.name_off[src]));
-        std::memcpy(o.data.data() + o.data_off[(size_t)i], r.data.data() + r.data_off[src],
+        copy_bytes(o.data.data() + o.data_off[(size_t)i], r.data.data() + r.data_off[src],
                     (size_t)(r.data_off[src + 1] - r.data_off[src]));
       }
     }
@@ -486,8 +495,10 @@ void preprocess(const Records& sorted, int k, int threads, Prep& p) {
           w2[0] = plo;
           w2[1] = phi;
         }
-        const uint32_t* src = tbuf[(size_t)owner[s]].data() + at[s];
-        std::memcpy(p.kp.data() + p.kp_off[s], src, sizeof(uint32_t) * (size_t)distinct[s]);
+        if (distinct[s] > 0) {  // (no profile at all: kp may have no storage)
+          const uint32_t* src = tbuf[(size_t)owner[s]].data() + at[s];
+          copy_bytes(p.kp.data() + p.kp_off[s], src, sizeof(uint32_t) * (size_t)distinct[s]);
+        }
       }
     }
   };

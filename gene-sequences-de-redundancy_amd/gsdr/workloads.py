@@ -21,6 +21,9 @@ SINGLE = {
     "c1": (1, 1001, 10000, 0.95, "mutated families: 10,000 records from 500 parents"),
     "c2": (2, 1002, 200000, 0.90, "mutated families: 200,000 records, log-normal median 1,200"),
     "c4": (4, 1004, 2000000, 0.95, "skewed single job: 2,000,000 records, Pareto(1.5) lengths"),
+    # a high-threshold job of the many-job and stream laws: nearly every record
+    # is its own representative (n/64 speculative steps at most)
+    "t99": (2, 1002, 177000, 0.99, "high-threshold job: 177,000 records, log-normal median 1,200"),
 }
 
 MANY_MEDIANS = (150.0, 400.0, 1500.0, 5000.0)

@@ -15,7 +15,8 @@ its digest, cluster it through the C ABI and compare the other two.
 
 Workloads:
   single   10,000-record families at t=0.95; 200,000 log-normal records at
-           t=0.85, 0.90, 0.95; 2,000,000 Pareto records at t=0.95 and 0.80
+           t=0.85, 0.90, 0.95; 2,000,000 Pareto records at t=0.95 and 0.80;
+           177,000 log-normal records at t=0.99
   many     the first 64 jobs of the 512-job workload (sizes log-uniform in
            [500, 50,000], medians {150, 400, 1500, 5000}, t in
            {.80, .85, .90, .95, .99})
@@ -48,7 +49,7 @@ from gsdr import workloads as W  # noqa: E402
 
 OUT = os.path.join(HERE, "full_hashes.json")
 
-SINGLE_RUNS = [("c1", 0.95), ("c2", 0.85), ("c2", 0.90), ("c2", 0.95), ("c4", 0.95), ("c4", 0.80)]
+SINGLE_RUNS = [("c1", 0.95), ("c2", 0.85), ("c2", 0.90), ("c2", 0.95), ("c4", 0.95), ("c4", 0.80), ("t99", 0.99)]
 MANY_COUNT = 64
 STREAM_COUNT = 64
 
