@@ -54,7 +54,11 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.smax = smax;
   L.fork = T.fork_shapes;
   L.filter_grid = c->cu_count * T.filter_blocks;
-  L.reps_filter_grid = std::max(16, std::min(njobs * 4, c->cu_count * 4));
+  // phase A holds at most 64 filter units per job (one representative table
+  // per block for long words): one block per unit, so the units' table
+  // builds run side by side (16 blocks in all took 35 us per step of a
+  // t=0.99 job, four units one after another in each block)
+  L.reps_filter_grid = std::max(16, std::min(njobs * 64, c->cu_count * 4));
   for (int sh = 0; sh < kNumShapes; ++sh) {
     L.align_grid[sh] = c->cu_count * align_blocks_for(T, sh);
     // phase A holds at most k_spec^2/2 pairs per job
@@ -115,6 +119,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.ra.stats = w.stats;
   L.ra.pair_budget = w.item_cap;
   L.ra.grid = c->cu_count * 2;
+  L.ra.reps_marked = 1;
   L.pigeon = T.pigeon;
   L.pigeon_grid = c->cu_count * 4;
   L.pa = PigeonArgs{w.jobs, w.items, w.item_cap, w.counters, smax, w.stats};

@@ -198,10 +198,13 @@ __global__ __launch_bounds__(256) void resolve_state_kernel(ResolveArgs a) {
   const int32_t* cand = J.cand[cur];
   const unsigned long long tm = S->tmask;
   // the true representatives are their own clusters (with one slot this is
-  // the round's representative, updateRepresentative, func.cpp:466-476)
-  for (unsigned long long m = tm; m; m &= m - 1) {
-    const int r = cand[__builtin_ctzll(m)];
-    J.cluster[r] = r;
+  // the round's representative, updateRepresentative, func.cpp:466-476);
+  // in the batched form resolve_reps_kernel has written them
+  if (!a.reps_marked) {
+    for (unsigned long long m = tm; m; m &= m - 1) {
+      const int r = cand[__builtin_ctzll(m)];
+      J.cluster[r] = r;
+    }
   }
   const int true_reps = __popcll(tm);
   const int absorbed = S->absorbed;

@@ -60,6 +60,8 @@ struct ResolveArgs {
   int32_t* jobs_left;       // jobs not done (device)
   unsigned long long* stats;
   int64_t pair_budget;      // alignment list capacity per shape (bounds k_spec * nc over the jobs)
+  int32_t reps_marked = 0;  // resolve_reps_kernel has already made the true representatives their own
+                            // clusters (batched form)
   int32_t grid;             // workgroups of the resolve tile kernels
 };
 
