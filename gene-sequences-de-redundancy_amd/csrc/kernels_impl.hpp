@@ -859,18 +859,20 @@ __device__ __forceinline__ uint32_t add_carry_in(uint32_t a, uint64_t cin) {
 // out. The code is fixed at compile time, so M & V is one three-input bit
 // operation on (V, lo plane, hi plane).
 template <int B, int W>
-__device__ __forceinline__ void row_sum_base(const uint32_t (&v)[W], const uint32_t (&tl)[W],
-                                             const uint32_t (&th)[W], uint32_t (&s)[W], uint32_t (&u)[W],
-                                             uint32_t& carry) {
+__device__ __forceinline__ void row_sum_base(uint32_t (&v)[W], const uint32_t (&tl)[W],
+                                             const uint32_t (&th)[W], uint32_t (&s)[W], uint32_t& carry) {
   carry = 0u;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const uint32_t ml = (B & 1) ? tl[w] : ~tl[w];
     const uint32_t mh = (B & 2) ? th[w] : ~th[w];
-    u[w] = v[w] & ml & mh;
+    const uint32_t u = v[w] & ml & mh;
     uint32_t co;
-    s[w] = __builtin_addc(v[w], u[w], carry, &co);
+    s[w] = __builtin_addc(v[w], u, carry, &co);
     carry = co;
+    // V is not needed again in its old form: keep V & ~M (= V ^ U) in its
+    // place for V' = S | (V & ~M), which saves W registers over keeping U
+    v[w] ^= u;
   }
   // the carry leaves the branch as a 0/1 register value: merging a lane-mask
   // value over the four branches would cost scalar mask arithmetic per row
@@ -886,8 +888,7 @@ __device__ __forceinline__ void row_sum_base(const uint32_t (&v)[W], const uint3
 // look-ahead runs, and the ripple through a lane's other words runs only
 // when some lane needs it.
 template <int W>
-__device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], const uint32_t (&u)[W],
-                                             uint32_t carry, int lane) {
+__device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], uint32_t carry, int lane) {
   constexpr uint64_t KTOP = 1ull << 63;
   if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) == 0) {
     // no first word is all ones: the carry in cannot ripple past it
@@ -915,7 +916,7 @@ __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W],
     }
   }
 #pragma unroll
-  for (int w = 0; w < W; ++w) v[w] = s[w] | (v[w] ^ u[w]);
+  for (int w = 0; w < W; ++w) v[w] = s[w] | v[w];  // v holds V & ~M here
 }
 
 // S = V + U over a lane's W words (word 0 lowest) as one v_add_co / v_addc
@@ -1369,19 +1370,29 @@ template <int W, bool SPLIT>
 __device__ __forceinline__ void load_window64(const Pair64& P, const Stage64& X, int lane, int b, int ob, int ow,
                                               uint32_t (&v)[W], uint32_t (&tl)[W], uint32_t (&th)[W]) {
   // selects rather than branches: writing the word arrays in divergent
-  // branches would move them to scratch memory
+  // branches would move them to scratch memory. Two passes (staged words
+  // from LDS, then new words from the store), so that each word's loads go
+  // straight to its registers: one pass held every word's LDS and store
+  // values at once, the register peak of the wide tiles.
+  const int base = b + lane * W - ob;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    const int g = b + lane * W + w;
-    const int rel = g - ob;
+    const int rel = base + w;
     const bool staged = rel < 64 * ow;
     const int ri = staged ? rel : 0;
     const uint32_t sv = X.v[ri], sl = X.tl[ri], sh = X.th[ri];
-    uint2 t = make_uint2(0u, 0u);
-    if (!staged && g < P.ntw) t = SPLIT ? text_word64(P, g) : P.pk[P.toff + g];
     v[w] = staged ? sv : ~0u;
-    tl[w] = staged ? sl : t.x;
-    th[w] = staged ? sh : t.y;
+    tl[w] = staged ? sl : 0u;
+    th[w] = staged ? sh : 0u;
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int g = b + lane * W + w;
+    if (g - ob >= 64 * ow && g < P.ntw) {
+      const uint2 t = SPLIT ? text_word64(P, g) : P.pk[P.toff + g];
+      tl[w] = t.x;
+      th[w] = t.y;
+    }
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -1428,14 +1439,14 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
       // code specialised for its base; the empty asm keeps the four match
       // masks from being hoisted out of the loop (4*W more registers).
       auto row = [&](uint32_t base) {
-        uint32_t s[W], u[W], carry;
+        uint32_t s[W], carry;
 #pragma unroll
         for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
-        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, u, carry);
-        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, u, carry);
-        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, u, carry);
-        else row_sum_base<3, W>(v, tl, th, s, u, carry);
-        row_finish64<W>(v, s, u, carry, lane);
+        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, carry);
+        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, carry);
+        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, carry);
+        else row_sum_base<3, W>(v, tl, th, s, carry);
+        row_finish64<W>(v, s, carry, lane);
       };
       if (W <= 8 && nrows == 32) {
         // a full block in chunks of 8 rows with constant bit offsets (the
@@ -1475,7 +1486,7 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
         row_masks64<W>(v, tl, th, ql, qh, lane);
       }
     }
-    if (lane == 0) cells += 64ull * W * 32ull * 32ull;
+    cells += 64ull * W * 32ull * 32ull;  // the same in every lane: a scalar register
 
     // ---- block end: zero counts per lane (all columns, and columns < n-R)
     const int jl = jb * 32 + nrows - 1;  // last row done
@@ -1495,7 +1506,7 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
     // columns < n-R left of the window are all counted: an upper bound
     const int dstar = frozen + (int)(tot >> 16);
     if (z >= c || dstar + R < c || R == 0) {
-      if (lane == 0) cdec += (unsigned long long)band_cells_rows(n, mq, c, jl + 1);
+      cdec += (unsigned long long)uniform64((int64_t)band_cells_rows(n, mq, c, jl + 1));
       return z >= c ? 1 : 0;
     }
     if constexpr (SPLIT) {
@@ -1538,7 +1549,7 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
       const int dead_col = (b + ndead * W) * 32;  // columns before it are dead after row jl
       if (dead_col > min(n - 1, jl + n - c)) {
         // every column of row jl's band is dead: no path of c matches exists
-        if (lane == 0) cdec += (unsigned long long)band_cells_rows(n, mq, c, jl + 1);
+        cdec += (unsigned long long)uniform64((int64_t)band_cells_rows(n, mq, c, jl + 1));
         return 0;
       }
       lo_col = max(lo_col, dead_col);
@@ -1556,12 +1567,12 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
         for (int off = 32; off > 0; off >>= 1) dz += __shfl_xor(dz, off);
         stage_window64<W>(X, lane, v, tl, th);
         frozen += uniform(dz);
-        if (w2 != W) {
-          S.b = nb; S.jb = jb; S.frozen = frozen; S.ob = b; S.ow = W; S.w = w2;
-          return -1;
-        }
-        load_window64<W, SPLIT>(P, X, lane, nb, b, W, v, tl, th);
-        b = nb;
+        // the window moves (and may change width): the caller re-enters at
+        // the new base, whose first act reloads the window from LDS (one
+        // place where the word arrays are defined keeps them in the same
+        // registers; a reload here made the compiler copy them at the join)
+        S.b = nb; S.jb = jb; S.frozen = frozen; S.ob = b; S.ow = W; S.w = w2;
+        return -1;
       }
     }
   }
@@ -1616,7 +1627,7 @@ __device__ inline void align_loop64(const AlignArgs& a) {
     const int cand = (rslot & kSwapFlag) ? text : query;  // the candidate owns the cutoff
     const int n = uniform(st.m[text]), mq = uniform(st.m[query]), c = uniform(st.bc[cand]) + 1;
     if (a.skip_absorbed && uniform64(J.acc[cand] & J.state->tmask & ((1ull << islot) - 1ull))) {
-      if (lane == 0) ++nskip;
+      ++nskip;  // per-wave counters are uniform: scalar registers
       continue;
     }
     int res = 0;
@@ -1638,8 +1649,8 @@ __device__ inline void align_loop64(const AlignArgs& a) {
       } else if (res) {
         atomicOr(&J.acc[cand], 1ull << islot);
       }
-      if (res) ++nacc; else ++nrej;
     }
+    if (res) ++nacc; else ++nrej;
   }
   if (lane == 0 && a.stats) {
     unsigned long long* sr = stat_row(a.stats);
@@ -1842,10 +1853,11 @@ __global__ __launch_bounds__(128) void align_split_kernel(AlignArgs a) {
 
 // Waves per SIMD the register allocation must allow (HIP's second launch
 // bound): 8 for the narrow one-word tiles (62 registers, no spills; lets a
-// grid of 8 blocks per CU be resident), 4 up to 10 words per lane, then 3
-// and 2 (measured: the widest tiles cannot fit more without spilling).
+// grid of 8 blocks per CU be resident), 4 up to 10 words per lane (128
+// registers: the per-wave counters in scalar registers, the window reloaded
+// in one place), 3 for 12 and 16 words; none spills.
 __host__ __device__ constexpr int align_waves_per_eu(int s, int w) {
-  return s < 64 ? 8 : w <= 8 ? 4 : w <= 12 ? 3 : 2;
+  return s < 64 ? 8 : w <= 10 ? 4 : 3;
 }
 
 template <int S, int W>
