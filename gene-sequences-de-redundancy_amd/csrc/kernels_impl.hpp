@@ -858,21 +858,26 @@ __device__ __forceinline__ uint32_t add_carry_in(uint32_t a, uint64_t cin) {
 // mask, U = V & M and the per-lane multi-word sum S = V + U with its carry
 // out. The code is fixed at compile time, so M & V is one three-input bit
 // operation on (V, lo plane, hi plane).
-template <int B, int W>
+//
+// With KEEPU, U stays in u[] for the second half, which then forms
+// V' = S | (V ^ U) in one three-input operation per word. Without it, V is
+// replaced by V & ~M (= V ^ U) here, one more operation per word but W
+// fewer registers: the widest tiles take that to keep their occupancy.
+template <int B, int W, bool KEEPU>
 __device__ __forceinline__ void row_sum_base(uint32_t (&v)[W], const uint32_t (&tl)[W],
-                                             const uint32_t (&th)[W], uint32_t (&s)[W], uint32_t& carry) {
+                                             const uint32_t (&th)[W], uint32_t (&s)[W], uint32_t (&u)[W],
+                                             uint32_t& carry) {
   carry = 0u;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const uint32_t ml = (B & 1) ? tl[w] : ~tl[w];
     const uint32_t mh = (B & 2) ? th[w] : ~th[w];
-    const uint32_t u = v[w] & ml & mh;
+    const uint32_t uw = v[w] & ml & mh;
     uint32_t co;
-    s[w] = __builtin_addc(v[w], u, carry, &co);
+    s[w] = __builtin_addc(v[w], uw, carry, &co);
     carry = co;
-    // V is not needed again in its old form: keep V & ~M (= V ^ U) in its
-    // place for V' = S | (V & ~M), which saves W registers over keeping U
-    v[w] ^= u;
+    if constexpr (KEEPU) u[w] = uw;
+    else v[w] ^= uw;
   }
   // the carry leaves the branch as a 0/1 register value: merging a lane-mask
   // value over the four branches would cost scalar mask arithmetic per row
@@ -887,8 +892,9 @@ __device__ __forceinline__ void row_sum_base(uint32_t (&v)[W], const uint32_t (&
 // ripple past the first word; otherwise the full generate/propagate
 // look-ahead runs, and the ripple through a lane's other words runs only
 // when some lane needs it.
-template <int W>
-__device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], uint32_t carry, int lane) {
+template <int W, bool KEEPU>
+__device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W], const uint32_t (&u)[W],
+                                             uint32_t carry, int lane) {
   constexpr uint64_t KTOP = 1ull << 63;
   if (__builtin_amdgcn_ballot_w64(s[0] == ~0u) == 0) {
     // no first word is all ones: the carry in cannot ripple past it
@@ -916,7 +922,7 @@ __device__ __forceinline__ void row_finish64(uint32_t (&v)[W], uint32_t (&s)[W],
     }
   }
 #pragma unroll
-  for (int w = 0; w < W; ++w) v[w] = s[w] | v[w];  // v holds V & ~M here
+  for (int w = 0; w < W; ++w) v[w] = KEEPU ? s[w] | (v[w] ^ u[w]) : s[w] | v[w];  // else v holds V & ~M
 }
 
 // S = V + U over a lane's W words (word 0 lowest) as one v_add_co / v_addc
@@ -1421,7 +1427,7 @@ struct Win64 {
 // Runs 32-row blocks of the pair with W words per lane, from the state in
 // S (window contents staged in X). Returns 1 (accept) or 0 (reject), or -1
 // after staging the window when the next block needs another width (S.w).
-template <int W, bool SPLIT>
+template <int W, bool SPLIT, bool KU>
 __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, int lane, Win64& S,
                                             unsigned long long& cells, unsigned long long& cdec) {
   const int n = P.n, mq = P.mq, c = P.c;
@@ -1439,14 +1445,14 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
       // code specialised for its base; the empty asm keeps the four match
       // masks from being hoisted out of the loop (4*W more registers).
       auto row = [&](uint32_t base) {
-        uint32_t s[W], carry;
+        uint32_t s[W], u[W], carry;
 #pragma unroll
         for (int w = 0; w < W; ++w) asm volatile("" : "+v"(tl[w]), "+v"(th[w]));
-        if (base == 0u) row_sum_base<0, W>(v, tl, th, s, carry);
-        else if (base == 1u) row_sum_base<1, W>(v, tl, th, s, carry);
-        else if (base == 2u) row_sum_base<2, W>(v, tl, th, s, carry);
-        else row_sum_base<3, W>(v, tl, th, s, carry);
-        row_finish64<W>(v, s, carry, lane);
+        if (base == 0u) row_sum_base<0, W, KU>(v, tl, th, s, u, carry);
+        else if (base == 1u) row_sum_base<1, W, KU>(v, tl, th, s, u, carry);
+        else if (base == 2u) row_sum_base<2, W, KU>(v, tl, th, s, u, carry);
+        else row_sum_base<3, W, KU>(v, tl, th, s, u, carry);
+        row_finish64<W, KU>(v, s, u, carry, lane);
       };
       if (W <= 8 && nrows == 32) {
         // a full block in chunks of 8 rows with constant bit offsets (the
@@ -1585,17 +1591,20 @@ __device__ __forceinline__ int run_blocks64(const Pair64& P, const Stage64& X, i
 template <int W, bool SPLIT>
 __device__ __forceinline__ int decide64(const Pair64& P, const Stage64& X, int lane, Win64& S,
                                         unsigned long long& cells, unsigned long long& cdec) {
+  // Tiles of up to 8 words a lane keep U through the row (one operation per
+  // word fewer); the wider tiles give those registers to occupancy
+  constexpr bool KU = W <= 8;
   while (true) {
     int r = -1;
     switch (S.w) {
-      case 16: if constexpr (W >= 16) r = run_blocks64<16, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 12: if constexpr (W >= 12) r = run_blocks64<12, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 10: if constexpr (W >= 10) r = run_blocks64<10, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 8: if constexpr (W >= 8) r = run_blocks64<8, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 6: if constexpr (W >= 6) r = run_blocks64<6, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 4: if constexpr (W >= 4) r = run_blocks64<4, SPLIT>(P, X, lane, S, cells, cdec); break;
-      case 3: if constexpr (W >= 3) r = run_blocks64<3, SPLIT>(P, X, lane, S, cells, cdec); break;
-      default: r = run_blocks64<2, SPLIT>(P, X, lane, S, cells, cdec); break;
+      case 16: if constexpr (W >= 16) r = run_blocks64<16, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 12: if constexpr (W >= 12) r = run_blocks64<12, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 10: if constexpr (W >= 10) r = run_blocks64<10, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 8: if constexpr (W >= 8) r = run_blocks64<8, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 6: if constexpr (W >= 6) r = run_blocks64<6, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 4: if constexpr (W >= 4) r = run_blocks64<4, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      case 3: if constexpr (W >= 3) r = run_blocks64<3, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
+      default: r = run_blocks64<2, SPLIT, KU>(P, X, lane, S, cells, cdec); break;
     }
     if (r != -1) return r;
   }
