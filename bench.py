@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--no-screen", action="store_true", help="turn the word-signature screen off")
     ap.add_argument("--split-rows", type=int, default=-1,
                     help="phase-A pairs with this many rows or more on two waves (0 = never; default: built in)")
+    ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
+                    help="set one scalar launch knob of gsdr_tuning (repeatable), e.g. align_blocks=8")
     ap.add_argument("--serial-shapes", action="store_true",
                     help="run the tile-shape kernels of a step phase one after another (profiling)")
     ap.add_argument("--jobs", action="store_true", help="many independent jobs in shared launches")
@@ -360,6 +362,15 @@ def main():
         tu.fork_shapes = 0
         eng.set_tuning(tu)
         tuning = dict(tuning or {}, fork_shapes=0)
+    for kv in args.knob:
+        # any scalar knob by name (the values are checked by gsdr_set_tuning)
+        name, _, val = kv.partition("=")
+        if name not in dict(gsdr.Tuning._fields_) or name in ("shape_mask", "shape_bias", "ms_default", "ms_tuned"):
+            sys.exit(f"bench.py: --knob {kv}: not a scalar launch knob")
+        tu = eng.tuning()
+        setattr(tu, name, int(val))
+        eng.set_tuning(tu)
+        tuning = dict(tuning or {}, **{name: int(val)})
     tune = tuning is not None
     ref = None
     for i in range(args.warmup):

@@ -36,7 +36,7 @@ struct Tuning {
   int swap_ratio = gsdr::kSwapRatio;
   int spec_max = gsdr::kMaxSpec;
   int filter_blocks = 8;
-  int align_blocks = 4;  // 0: each shape kernel's resident blocks per CU
+  int align_blocks = -1;  // -1: by the call (align_blocks_for); 0: each shape kernel's resident blocks per CU
   int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
   int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
   int pigeon = 0;        // exact pigeonhole prefilter between the filter and the alignment
@@ -200,16 +200,28 @@ struct StepLaunch {
   ScreenArgs sa{};
 };
 StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
-                            int32_t* jobs_left, const Tuning& T = Tuning());
-// Alignment workgroups per CU for a tile shape under knobs T; 0 = the
-// shape kernel's resident blocks per CU (registers and LDS decide it).
-// Measured with 8 per CU against 4: many short pairs gain (200,000 records,
-// t=0.90: 736 -> 638 ms per clustering), while a job whose long-pair kernel
-// ends each step loses (2,000,000 records, t=0.95: 473 -> 510 ms: that
-// kernel then shares the CUs with more blocks of the other shapes), so the
-// default is 4 and the tuner picks per workload.
-inline int align_blocks_for(const Tuning& T, int shape) {
-  return T.align_blocks > 0 ? T.align_blocks : gsdr::align_blocks_per_cu(shape);
+                            int32_t* jobs_left, bool busy, const Tuning& T = Tuning());
+// Alignment workgroups per CU for tile shape `shape` under knobs T, in a
+// call whose widest shape is smax. A positive knob is the count; 0 is each
+// shape kernel's resident blocks per CU (registers and LDS decide it); -1,
+// the default, takes the resident blocks for calls of many aligned pairs
+// per step (`busy`: at least kBusyBases and no threshold above
+// kBusyThreshold, so that most pairs pass the filters) where no wide-tile
+// kernel (64x6 and up) runs, and 4 otherwise. Measured on 200,000 records of
+// median 1,200 bases, ms per clustering with 4 and with the resident count:
+// t=0.85 99.8 / 93.3, t=0.90 550 / 493, t=0.95 250 / 267, t=0.99 812 / 853
+// (at high t the steps are many and small, and each step's launches pay for
+// blocks that find no work; 10,000 records at t=0.95: 30.6 / 32.7); where
+// a long-pair kernel ends each step the other shapes' extra blocks share
+// its CUs (2,000,000 records, t=0.95: 375 / 377).
+constexpr int64_t kBusyBases = 32ll << 20;
+constexpr float kBusyThreshold = 0.90f;
+constexpr int kMaxShortShape = 6;  // 64x4
+inline bool busy_call(int64_t bases, float tmax) { return bases >= kBusyBases && tmax <= kBusyThreshold; }
+inline int align_blocks_for(const Tuning& T, int shape, int smax, bool busy) {
+  if (T.align_blocks > 0) return T.align_blocks;
+  if (T.align_blocks == 0 || (busy && smax <= kMaxShortShape)) return gsdr::align_blocks_per_cu(shape);
+  return 4;
 }
 // The knobs a call runs with: the context's tuned ones when asked for and present.
 inline Tuning tuning_for(const gsdr_ctx* c, bool want) { return want && c->has_tuning ? c->tuned : Tuning(); }

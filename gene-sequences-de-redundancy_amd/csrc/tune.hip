@@ -92,7 +92,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
       {&Tuning::list_factor, {8, 16}},
       {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
       {&Tuning::spec_max, {16, 32}},
-      {&Tuning::align_blocks, {2, 8, 0}},
+      {&Tuning::align_blocks, {2, 4, 8, 0}},
       {&Tuning::filter_blocks, {4, 16}},
       {&Tuning::fork_shapes, {0}},
       {&Tuning::split_rows, {0, 1024}},
