@@ -135,6 +135,23 @@ def parallel_map(fn, items, threads=16):
         return list(ex.map(fn, items))
 
 
+def apply_knobs(eng, args, tuning=None):
+    """Sets each --knob NAME=VALUE on the engine's tuned knobs; returns the
+    dict of knobs set (None when none)."""
+    import gsdr
+
+    for kv in args.knob:
+        # any scalar knob by name (the values are checked by gsdr_set_tuning)
+        name, _, val = kv.partition("=")
+        if name not in dict(gsdr.Tuning._fields_) or name in ("shape_mask", "shape_bias", "ms_default", "ms_tuned"):
+            sys.exit(f"bench.py: --knob {kv}: not a scalar launch knob")
+        tu = eng.tuning()
+        setattr(tu, name, int(val))
+        eng.set_tuning(tu)
+        tuning = dict(tuning or {}, **{name: int(val)})
+    return tuning
+
+
 def bench_jobs(args):
     """Many independent jobs (log-uniform sizes, per-job median length and
     threshold) clustered together in shared batched launches."""
@@ -163,11 +180,13 @@ def bench_jobs(args):
     print(f"# loaded {bases / 1e6:.1f} Mbp in {load_s:.1f} s, HBM {sum(s.footprint for s in stores) / 2**30:.2f} GiB",
           file=sys.stderr, flush=True)
     ts = [s["t"] for s in specs]
-    eng.cluster_jobs(stores, ts)  # warmup
+    tuning = apply_knobs(eng, args)
+    tune = tuning is not None
+    eng.cluster_jobs(stores, ts, tune=tune)  # warmup
     times = []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        outs, st = eng.cluster_jobs(stores, ts)
+        outs, st = eng.cluster_jobs(stores, ts, tune=tune)
         times.append(time.perf_counter() - t0)
         print(f"# step {i}: {times[-1] * 1e3:.1f} ms, steps {st.rounds}, reps {st.reps}", file=sys.stderr, flush=True)
     # each job alone must give the same clusters (spot check)
@@ -189,7 +208,7 @@ def bench_jobs(args):
                       "records": int(sum(len(r) for r in recs)), "bases": int(bases)},
            "plain_sequential_s": round(plain_s, 3) if plain_s else None,
            "batched_over_plain": round(plain_s / (ms / 1e3), 3) if plain_s else None,
-           "alone_equals_mix_sample": bool(same), "reps": int(st.reps)}
+           "alone_equals_mix_sample": bool(same), "reps": int(st.reps), "tuned": tuning}
     for s in stores:
         s.free()
     eng.close()
@@ -362,15 +381,7 @@ def main():
         tu.fork_shapes = 0
         eng.set_tuning(tu)
         tuning = dict(tuning or {}, fork_shapes=0)
-    for kv in args.knob:
-        # any scalar knob by name (the values are checked by gsdr_set_tuning)
-        name, _, val = kv.partition("=")
-        if name not in dict(gsdr.Tuning._fields_) or name in ("shape_mask", "shape_bias", "ms_default", "ms_tuned"):
-            sys.exit(f"bench.py: --knob {kv}: not a scalar launch knob")
-        tu = eng.tuning()
-        setattr(tu, name, int(val))
-        eng.set_tuning(tu)
-        tuning = dict(tuning or {}, **{name: int(val)})
+    tuning = apply_knobs(eng, args, tuning)
     tune = tuning is not None
     ref = None
     for i in range(args.warmup):

@@ -138,7 +138,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   int smax = 0, timing = 0, spec = 0, tune = 0;
   unsigned kmask = 0;
   int64_t total_n = 0, total_bases = 0, max_n = 0;
-  float tmax = 0.f;
+  int64_t low_bases = 0;
   // host bookkeeping: on the stack for one job (the no-allocation entry)
   JobDev job1[1];
   std::vector<JobDev> jobs_many((size_t)(njobs > 1 ? njobs : 0));
@@ -157,7 +157,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     max_n = std::max<int64_t>(max_n, s.n);
     timing |= params[j].timing;
     spec = std::max(spec, params[j].spec_reps);
-    tmax = std::max(tmax, params[j].threshold);
+    low_bases += low_t_bases(s.bases, params[j].threshold);
     tune |= params[j].tune;
   }
   const Tuning T = tuning_for(c, tune != 0);
@@ -192,7 +192,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs, njobs, k_max, w.jobs_left, q), "init");
 
-  StepLaunch L = make_step_launch(c, w, njobs, smax, kmask, w.jobs_left, busy_call(total_bases, tmax), T);
+  StepLaunch L = make_step_launch(c, w, njobs, smax, kmask, w.jobs_left, busy_call(low_bases), T);
   // the speculation bound follows this call's capacity, not a larger
   // workspace left by an earlier call
   L.ra.pair_budget = std::min<int64_t>(item_cap, w.item_cap);

@@ -295,7 +295,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       aa[sh].win_period = T.win_period;
       const int segs = 64 / kShapes[sh].s;
       const int waves = (nc + segs - 1) / segs;
-      grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh, smax, busy_call(s.bases, p.threshold))));
+      grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh, smax, busy_call(low_t_bases(s.bases, p.threshold)))));
       ++launches;
       ++align_launches;
     }

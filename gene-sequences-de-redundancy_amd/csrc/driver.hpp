@@ -205,19 +205,25 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
 // call whose widest shape is smax. A positive knob is the count; 0 is each
 // shape kernel's resident blocks per CU (registers and LDS decide it); -1,
 // the default, takes the resident blocks for calls of many aligned pairs
-// per step (`busy`: at least kBusyBases and no threshold above
-// kBusyThreshold, so that most pairs pass the filters) where no wide-tile
-// kernel (64x6 and up) runs, and 4 otherwise. Measured on 200,000 records of
-// median 1,200 bases, ms per clustering with 4 and with the resident count:
-// t=0.85 99.8 / 93.3, t=0.90 550 / 493, t=0.95 250 / 267, t=0.99 812 / 853
-// (at high t the steps are many and small, and each step's launches pay for
-// blocks that find no work; 10,000 records at t=0.95: 30.6 / 32.7); where
-// a long-pair kernel ends each step the other shapes' extra blocks share
-// its CUs (2,000,000 records, t=0.95: 375 / 377).
+// per step (`busy`: at least kBusyBases in jobs of threshold at most
+// kBusyThreshold, where most pairs pass the filters) where no wide-tile
+// kernel (64x6 and up) runs, and 4 otherwise. Measured, ms per clustering
+// with 4 and with the resident count: 200,000 records of median 1,200
+// bases t=0.85 99.8 / 93.3, t=0.90 550 / 493, t=0.95 250 / 267, t=0.99
+// 812 / 853 (at high t the steps are many and small, and each step's
+// launches pay for blocks that find no work; 10,000 records at t=0.95: 30.6
+// / 32.7); 512 jobs of 500..10,000 records with thresholds 0.80..0.99 in
+// shared launches 3,857 / 3,553 (the low-threshold jobs make every shared
+// step large); where a long-pair kernel ends each step the other shapes'
+// extra blocks share its CUs (2,000,000 records, t=0.95: 375 / 377, t=0.90:
+// 259 / 263, t=0.80: 183 / 185).
 constexpr int64_t kBusyBases = 32ll << 20;
 constexpr float kBusyThreshold = 0.90f;
 constexpr int kMaxShortShape = 6;  // 64x4
-inline bool busy_call(int64_t bases, float tmax) { return bases >= kBusyBases && tmax <= kBusyThreshold; }
+// low_bases: the bases of the call's jobs whose threshold is at most
+// kBusyThreshold (in shared launches they make every step large)
+inline bool busy_call(int64_t low_bases) { return low_bases >= kBusyBases; }
+inline int64_t low_t_bases(int64_t bases, float t) { return t <= kBusyThreshold ? bases : 0; }
 inline int align_blocks_for(const Tuning& T, int shape, int smax, bool busy) {
   if (T.align_blocks > 0) return T.align_blocks;
   if (T.align_blocks == 0 || (busy && smax <= kMaxShortShape)) return gsdr::align_blocks_per_cu(shape);

@@ -338,9 +338,9 @@ class Engine:
                                       len(pr), shape, _i32p(out), C.byref(cells)), "align_pairs")
         return out.astype(bool), cells.value
 
-    def cluster_jobs(self, stores, thresholds, mode=MODE_BATCHED, spec_reps=0, drop_filters=None):
+    def cluster_jobs(self, stores, thresholds, mode=MODE_BATCHED, spec_reps=0, drop_filters=None, tune=False):
         """Clusters several stores in shared launches; drop_filters: one flag
-        per job (the drop mode), or None."""
+        per job (the drop mode), or None; tune: the context's tuned knobs."""
         n = len(stores)
         params = (Params * n)()
         for i, t in enumerate(thresholds):
@@ -349,6 +349,7 @@ class Engine:
             params[i].mode = mode
             params[i].spec_reps = spec_reps
             params[i].drop_filters = 1 if drop_filters is not None and drop_filters[i] else 0
+            params[i].tune = 1 if tune else 0
         outs = [np.zeros(s.n, dtype=np.int32) for s in stores]
         handles = (C.c_void_p * n)(*[s.h.value for s in stores])
         ptrs = (C.POINTER(C.c_int32) * n)(*[_i32p(o) for o in outs])
