@@ -94,8 +94,8 @@ typedef struct gsdr_tuning {
   int32_t filter_blocks;  /* filter workgroups per CU */
   int32_t align_blocks;   /* alignment workgroups per CU, per tile shape; 0 = each shape
                              kernel's resident blocks per CU; -1 (default) = those when the
-                             call holds at least 32 Mi bases, no threshold above 0.90 and no
-                             text that needs a tile wider than 64x4, else 4 */
+                             call's jobs of threshold <= 0.90 hold at least 32 Mi bases and
+                             no text needs a tile wider than 64x4, else 4 */
   int32_t trials;         /* autotune: clusterings timed */
   int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
   int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
