@@ -48,11 +48,11 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
 }
 
 StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
-                            int32_t* jobs_left, bool busy, const Tuning& T) {
+                            int32_t* jobs_left, GridMode mode, const Tuning& T) {
   StepLaunch L;
   L.kmask = kmask;
   L.smax = smax;
-  L.fork = T.fork_shapes;
+  L.fork = fork_for(T, mode);
   L.filter_grid = c->cu_count * T.filter_blocks;
   // phase A holds at most 64 filter units per job (one representative table
   // per block for long words): one block per unit, so the units' table
@@ -60,7 +60,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   // t=0.99 job, four units one after another in each block)
   L.reps_filter_grid = std::max(16, std::min(njobs * 64, c->cu_count * 4));
   for (int sh = 0; sh < kNumShapes; ++sh) {
-    L.align_grid[sh] = c->cu_count * align_blocks_for(T, sh, smax, busy);
+    L.align_grid[sh] = c->cu_count * align_blocks_for(T, sh, mode);
     // phase A holds at most k_spec^2/2 pairs per job
     const int segs = 64 / kShapes[sh].s;
     L.reps_grid[sh] = std::max(1, std::min(c->cu_count * 4, (njobs * kMaxSpec * kMaxSpec / 2 / segs + 3) / 4));
@@ -139,6 +139,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   unsigned kmask = 0;
   int64_t total_n = 0, total_bases = 0, max_n = 0;
   int64_t low_bases = 0;
+  float tmin = 1.f;
   // host bookkeeping: on the stack for one job (the no-allocation entry)
   JobDev job1[1];
   std::vector<JobDev> jobs_many((size_t)(njobs > 1 ? njobs : 0));
@@ -158,6 +159,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     timing |= params[j].timing;
     spec = std::max(spec, params[j].spec_reps);
     low_bases += low_t_bases(s.bases, params[j].threshold);
+    tmin = std::min(tmin, params[j].threshold);
     tune |= params[j].tune;
   }
   const Tuning T = tuning_for(c, tune != 0);
@@ -192,7 +194,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   HIP_TRY(hipMemcpyAsync(w.jobs_left, &left, sizeof(int32_t), hipMemcpyHostToDevice, q), "jobs_left");
   HIP_TRY(launch_init_jobs(w.jobs, jobs, njobs, k_max, w.jobs_left, q), "init");
 
-  StepLaunch L = make_step_launch(c, w, njobs, smax, kmask, w.jobs_left, busy_call(low_bases), T);
+  StepLaunch L = make_step_launch(c, w, njobs, smax, kmask, w.jobs_left, grid_mode(low_bases, tmin, smax), T);
   // the speculation bound follows this call's capacity, not a larger
   // workspace left by an earlier call
   L.ra.pair_budget = std::min<int64_t>(item_cap, w.item_cap);

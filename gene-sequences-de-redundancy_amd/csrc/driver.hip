@@ -249,6 +249,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
   int nc = n, first = 0;
   bool done = n == 0;
   const Tuning T = tuning_for(c, p.tune != 0);
+  const GridMode mode = grid_mode(low_t_bases(s.bases, p.threshold), p.threshold, max_shape_for(s.max_m));
   while (!done) {
     ++rounds;
     const int units = (nc + kFilterChunk - 1) / kFilterChunk;
@@ -295,11 +296,11 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       aa[sh].win_period = T.win_period;
       const int segs = 64 / kShapes[sh].s;
       const int waves = (nc + segs - 1) / segs;
-      grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh, smax, busy_call(low_t_bases(s.bases, p.threshold)))));
+      grids[sh] = std::max(1, std::min((waves + 3) / 4, c->cu_count * align_blocks_for(T, sh, mode)));
       ++launches;
       ++align_launches;
     }
-    HIP_TRY(launch_align_shapes(c, aa, smax, grids, q, T.fork_shapes), "align launch");
+    HIP_TRY(launch_align_shapes(c, aa, smax, grids, q, fork_for(T, mode)), "align launch");
     if (p.timing) HIP_TRY(hipEventRecord(ev[2], q), "event");
     ResolveArgs ra;
     ra.jobs = w.jobs;

@@ -37,7 +37,7 @@ struct Tuning {
   int spec_max = gsdr::kMaxSpec;
   int filter_blocks = 8;
   int align_blocks = -1;  // -1: by the call (align_blocks_for); 0: each shape kernel's resident blocks per CU
-  int fork_shapes = 1;   // tile-shape kernels concurrent on side streams (1) or serial (0)
+  int fork_shapes = -1;  // tile-shape kernels concurrent on side streams (1), serial (0), by the call (-1)
   int list_factor = 64;  // alignment list capacity per shape: list_factor items per sequence
   int pigeon = 0;        // exact pigeonhole prefilter between the filter and the alignment
   int screen = 1;        // word-signature screen before the phase-B filter (k >= 6)
@@ -199,36 +199,53 @@ struct StepLaunch {
   int screen_grid = 1;
   ScreenArgs sa{};
 };
-StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
-                            int32_t* jobs_left, bool busy, const Tuning& T = Tuning());
-// Alignment workgroups per CU for tile shape `shape` under knobs T, in a
-// call whose widest shape is smax. A positive knob is the count; 0 is each
-// shape kernel's resident blocks per CU (registers and LDS decide it); -1,
-// the default, takes the resident blocks for calls of many aligned pairs
-// per step (`busy`: at least kBusyBases in jobs of threshold at most
-// kBusyThreshold, where most pairs pass the filters) where no wide-tile
-// kernel (64x6 and up) runs, and 4 otherwise. Measured, ms per clustering
-// with 4 and with the resident count: 200,000 records of median 1,200
-// bases t=0.85 99.8 / 93.3, t=0.90 550 / 493, t=0.95 250 / 267, t=0.99
-// 812 / 853 (at high t the steps are many and small, and each step's
-// launches pay for blocks that find no work; 10,000 records at t=0.95: 30.6
-// / 32.7); 512 jobs of 500..10,000 records with thresholds 0.80..0.99 in
-// shared launches 3,857 / 3,553 (the low-threshold jobs make every shared
-// step large); where a long-pair kernel ends each step the other shapes'
-// extra blocks share its CUs (2,000,000 records, t=0.95: 375 / 377, t=0.90:
-// 259 / 263, t=0.80: 183 / 185).
+// How a call's steps are launched when the knobs say "by the call" (-1):
+//   busy: many aligned pairs per step, i.e. at least kBusyBases in jobs of
+//     threshold at most kBusyThreshold (most pairs pass the filters), and
+//     no text that needs a tile wider than 64x4 (kMaxShortShape): each
+//     tile-shape kernel gets its resident blocks per CU;
+//   latency: otherwise, when no wide-tile kernel runs or every job has a
+//     threshold of at least kLatencyThreshold (few pairs per step, many
+//     steps): the tile-shape kernels run one after another with 2 blocks
+//     per CU, which saves the fork/join waits across the hardware queues
+//     and the launches of blocks that find no work;
+//   normal: 4 blocks per CU, tile-shape kernels concurrent.
+// Measured, ms per clustering (4 blocks concurrent / resident / serial with
+// 2 blocks): 200,000 records of median 1,200 bases t=0.85 99.8 / 93.3 / -,
+// t=0.90 550 / 493 / -, t=0.95 250 / 267 / 242, t=0.99 809 / 853 / 671;
+// the 177,000-record t=0.99 job 705 / 740 / 580; 10,000 records t=0.95
+// 30.3 / 32.7 / 26.2; 512 jobs of 500..10,000 records (thresholds
+// 0.80..0.99) in shared launches 3,857 / 3,553 / -; 2,000,000 records
+// (texts to 30,000 bases) t=0.95 375 / 377 / 571, t=0.90 259 / 263 / -,
+// t=0.80 183 / 185 / -, t=0.99 9,566 / - / 9,222 (where a long-pair kernel
+// ends each step, the other shapes' extra blocks share its CUs, and the
+// serial form leaves the chip to that kernel alone).
+enum class GridMode { kNormal, kBusy, kLatency };
 constexpr int64_t kBusyBases = 32ll << 20;
 constexpr float kBusyThreshold = 0.90f;
+constexpr float kLatencyThreshold = 0.99f;
 constexpr int kMaxShortShape = 6;  // 64x4
-// low_bases: the bases of the call's jobs whose threshold is at most
-// kBusyThreshold (in shared launches they make every step large)
-inline bool busy_call(int64_t low_bases) { return low_bases >= kBusyBases; }
-inline int64_t low_t_bases(int64_t bases, float t) { return t <= kBusyThreshold ? bases : 0; }
-inline int align_blocks_for(const Tuning& T, int shape, int smax, bool busy) {
-  if (T.align_blocks > 0) return T.align_blocks;
-  if (T.align_blocks == 0 || (busy && smax <= kMaxShortShape)) return gsdr::align_blocks_per_cu(shape);
-  return 4;
+// low_bases: bases of the jobs of threshold <= kBusyThreshold; tmin: the
+// lowest threshold of the call; smax: its widest tile shape
+inline GridMode grid_mode(int64_t low_bases, float tmin, int smax) {
+  if (smax <= kMaxShortShape && low_bases >= kBusyBases) return GridMode::kBusy;
+  if (smax <= kMaxShortShape || tmin >= kLatencyThreshold) return GridMode::kLatency;
+  return GridMode::kNormal;
 }
+inline int64_t low_t_bases(int64_t bases, float t) { return t <= kBusyThreshold ? bases : 0; }
+// Alignment workgroups per CU of tile shape `shape`: a positive knob is the
+// count, 0 each shape kernel's resident blocks per CU, -1 by the call.
+inline int align_blocks_for(const Tuning& T, int shape, GridMode m) {
+  if (T.align_blocks > 0) return T.align_blocks;
+  if (T.align_blocks == 0 || m == GridMode::kBusy) return gsdr::align_blocks_per_cu(shape);
+  return m == GridMode::kLatency ? 2 : 4;
+}
+// Tile-shape kernels concurrent on side streams (1) or one after another (0).
+inline int fork_for(const Tuning& T, GridMode m) {
+  return T.fork_shapes >= 0 ? T.fork_shapes : (m == GridMode::kLatency ? 0 : 1);
+}
+StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
+                            int32_t* jobs_left, GridMode mode, const Tuning& T = Tuning());
 // The knobs a call runs with: the context's tuned ones when asked for and present.
 inline Tuning tuning_for(const gsdr_ctx* c, bool want) { return want && c->has_tuning ? c->tuned : Tuning(); }
 // The shape tables and window period of T, as the C ABI reports them.

@@ -197,9 +197,11 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
     int smax = 0;
     unsigned kmask = 0;
     int64_t low_bases = 0;
+    float tmin = 1.f;
     for (int i = 0; i < nj; ++i) {
       table[i] = srv->running[i]->store->job(srv->running[i]->params);
       low_bases += low_t_bases(srv->running[i]->store->bases, srv->running[i]->params.threshold);
+      tmin = std::min(tmin, srv->running[i]->params.threshold);
       smax = std::max(smax, max_shape_for(srv->running[i]->store->max_m));
       kmask |= 1u << srv->running[i]->store->k;
     }
@@ -216,7 +218,7 @@ int gsdr_serve_step(gsdr_server* srv, int32_t* running_out) {
     // recapture when the table changed, or when another call on the context
     // reallocated the workspace the graph's kernels point into
     if (!srv->exec || srv->graph_jobs != srv->running || srv->graph_ws != c->ws.generation) {
-      const StepLaunch L = make_step_launch(c, c->ws, nj, smax, kmask, nullptr, busy_call(low_bases));
+      const StepLaunch L = make_step_launch(c, c->ws, nj, smax, kmask, nullptr, grid_mode(low_bases, tmin, smax));
       std::lock_guard<std::mutex> dk(srv->dev_mu);
       if ((e = capture_steps(srv, L, q)) != hipSuccess) return hip_fail(e, "serve capture");
     }

@@ -90,6 +90,9 @@ class Tuning(C.Structure):
     def __init__(self, **kw):
         # the signature screen is on unless a caller turns it off
         kw.setdefault("screen", 1)
+        # grids and shape concurrency chosen by the call (the library default)
+        kw.setdefault("align_blocks", -1)
+        kw.setdefault("fork_shapes", -1)
         super().__init__(**kw)
 
     def as_dict(self):

@@ -93,11 +93,15 @@ typedef struct gsdr_tuning {
   int32_t spec_max;       /* batched: cap on speculative representatives per step (1..64) */
   int32_t filter_blocks;  /* filter workgroups per CU */
   int32_t align_blocks;   /* alignment workgroups per CU, per tile shape; 0 = each shape
-                             kernel's resident blocks per CU; -1 (default) = those when the
-                             call's jobs of threshold <= 0.90 hold at least 32 Mi bases and
-                             no text needs a tile wider than 64x4, else 4 */
+                             kernel's resident blocks per CU; -1 (default) = by the call: those
+                             when its jobs of threshold <= 0.90 hold at least 32 Mi bases and no
+                             text needs a tile wider than 64x4; 2 where fork_shapes -1 runs the
+                             shapes one after another; else 4 */
   int32_t trials;         /* autotune: clusterings timed */
-  int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another */
+  int32_t fork_shapes;    /* 1 = tile-shape kernels concurrent on side streams, 0 = one after another,
+                             -1 (default) = by the call: one after another where few pairs make a
+                             step (no tile wider than 64x4 needed, or every threshold >= 0.99) and
+                             the call is not one of many short aligned pairs */
   int32_t list_factor;    /* batched: alignment list capacity per tile shape, in items per
                              sequence (default 64); it also bounds k_spec * candidates */
   int32_t pigeon;         /* 1 = exact pigeonhole prefilter before the alignment (sound; default 0) */
