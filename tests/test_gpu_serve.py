@@ -1,9 +1,10 @@
 """Serving: jobs admitted while others run share launches; each result must be
-the same as the job clustered alone."""
+the same as the job clustered alone, and (staggered admission) the oracle's."""
 import numpy as np
 import pytest
 
 import gsdr
+import oracle_ffi as O
 from gsdr import serve
 
 pytestmark = pytest.mark.gpu
@@ -23,9 +24,19 @@ def alone(engine, recs, t):
     return cl
 
 
-def test_staggered_admission_matches_alone(engine):
-    recs = [make(c, s, n) for c, s, n, _ in SPECS]
-    ref = [alone(engine, r, t) for r, (_, _, _, t) in zip(recs, SPECS)]
+def test_staggered_admission_matches_alone(engine, tmp_path):
+    recs, ref = [], []
+    for c, s, n, t in SPECS:
+        r = gsdr.Records.generate(c, s, n)
+        fa = str(tmp_path / f"job_{s}.fasta")
+        r.write_fasta(fa)
+        r.sort()
+        recs.append(r)
+        # the solo run is checked against the oracle on the same file
+        cl = alone(engine, r, t)
+        ocl, _, _ = O.cluster_file(fa, t)
+        assert np.array_equal(cl, ocl), f"job seed {s} t={t}: solo run differs from the oracle"
+        ref.append(cl)
     srv = serve.Server(engine, hbm_budget=8 << 30, max_slots=8)
     try:
         ids = {}
