@@ -14,7 +14,8 @@ through the C ABI and compares the other two digests:
   stream        64 jobs of the serving stream through the continuous-batching
                 loop (Poisson arrivals, admission by slot and HBM budget)
   drop-in       the `cluster` binary (result.fasta, --membership) and the
-                reference-side binding of INTEGRATION.md, run as programs
+                reference-side binding example of the integration guide, run
+                as programs
 """
 import hashlib
 import json
@@ -179,8 +180,9 @@ def test_cli_drop_in_matches_oracle(tmp_path, tmp_path_factory, key, mode):
 
 
 def test_integration_binding_runs(tmp_path, tmp_path_factory):
-    """INTEGRATION.md's reference-side main.cpp, built against libgsdr.so
-    and run on the 10,000-record input, writes the oracle's result.fasta."""
+    """The integration guide's reference-side main.cpp, built against
+    libgsdr.so and run on the 10,000-record input, writes the oracle's
+    result.fasta."""
     from test_host import build_integration_example
 
     e = FIX["single"]["c1_0.95"]

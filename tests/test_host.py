@@ -225,7 +225,8 @@ def test_parse_options_follows_check_option():
 
 
 def integration_example():
-    """The C++ block of INTEGRATION.md section 2 (the reference-side binding)."""
+    """The C++ block of section 2 of the integration guide (the
+    reference-side binding), as the guide's file holds it."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     sec = text[text.index("## 2."):text.index("## 3.")]
     return sec[sec.index("```cpp") + len("```cpp"):sec.index("```", sec.index("```cpp") + 6)]
