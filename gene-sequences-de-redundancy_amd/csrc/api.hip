@@ -355,7 +355,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
     c->has_tuning = false;
     return GSDR_OK;
   }
-  if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || t->filter_blocks < 1 ||
+  if (t->swap_ratio < 0 || t->spec_max < 1 || t->spec_max > kMaxSpec || (t->filter_blocks < 1 && t->filter_blocks != -1) ||
       t->filter_blocks > 64 || t->align_blocks < -1 || t->align_blocks > 64 ||
       t->fork_shapes < -1 || t->fork_shapes > 1 || t->list_factor < 1 || t->list_factor > 256 ||
       (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1) || t->win_period < 0 ||

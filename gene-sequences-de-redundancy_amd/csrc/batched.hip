@@ -53,7 +53,7 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.kmask = kmask;
   L.smax = smax;
   L.fork = fork_for(T, mode);
-  L.filter_grid = c->cu_count * T.filter_blocks;
+  L.filter_grid = c->cu_count * filter_blocks_for(T, mode);
   // phase A holds at most 64 filter units per job (one representative table
   // per block for long words): one block per unit, so the units' table
   // builds run side by side (16 blocks in all took 35 us per step of a

@@ -272,7 +272,7 @@ int cluster_plain(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p, int32_t* out
       ++launches;
       ++filter_launches;
     }
-    HIP_TRY(launch_filter(fa, 1u << k, std::max(1, std::min(units, c->cu_count * T.filter_blocks)), q),
+    HIP_TRY(launch_filter(fa, 1u << k, std::max(1, std::min(units, c->cu_count * filter_blocks_for(T, mode))), q),
             "filter launch");
     ++launches;
     ++filter_launches;

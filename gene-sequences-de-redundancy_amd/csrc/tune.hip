@@ -93,7 +93,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
       {&Tuning::swap_ratio, {2, 3, 6, 8, 0}},
       {&Tuning::spec_max, {16, 32}},
       {&Tuning::align_blocks, {2, 4, 8, 0}},
-      {&Tuning::filter_blocks, {4, 16}},
+      {&Tuning::filter_blocks, {4, 8, 16, 32}},
       {&Tuning::fork_shapes, {0, 1}},
       {&Tuning::split_rows, {0, 1024}},
   };

@@ -92,6 +92,7 @@ class Tuning(C.Structure):
         kw.setdefault("screen", 1)
         # grids and shape concurrency chosen by the call (the library default)
         kw.setdefault("align_blocks", -1)
+        kw.setdefault("filter_blocks", -1)
         kw.setdefault("fork_shapes", -1)
         super().__init__(**kw)
 

@@ -91,7 +91,8 @@ typedef struct gsdr_stats {
 typedef struct gsdr_tuning {
   int32_t swap_ratio;     /* align with the roles swapped when n_rep >= swap_ratio * m_cand (0 = never) */
   int32_t spec_max;       /* batched: cap on speculative representatives per step (1..64) */
-  int32_t filter_blocks;  /* filter workgroups per CU */
+  int32_t filter_blocks;  /* filter workgroups per CU (-1, the default: by the call, 32 where
+                             the shapes run concurrently with 4 align workgroups per CU, else 8) */
   int32_t align_blocks;   /* alignment workgroups per CU, per tile shape; 0 = each shape
                              kernel's resident blocks per CU; -1 (default) = by the call: those
                              when its jobs of threshold <= 0.90 hold at least 32 Mi bases and no

@@ -24,7 +24,7 @@ def test_every_knob_setting_gives_the_oracle_clusters(engine, tmp_path, law, n, 
     store = engine.load(r, gsdr.auto_word_length(t))
     try:
         settings = [(2, 64, 8, 4, 1, 8), (0, 64, 8, 0, 0, 64), (8, 1, 1, 1, 1, 1), (3, 16, 16, 2, 0, 32),
-                    (1, 7, 2, 8, 1, 2), (4, 64, 8, -1, -1, 64)]
+                    (1, 7, 2, 8, 1, 2), (4, 64, -1, -1, -1, 64)]
         for swap, spec, fb, ab, fork, lf in settings:
             tu = gsdr.Tuning(swap_ratio=swap, spec_max=spec, filter_blocks=fb, align_blocks=ab, fork_shapes=fork,
                              list_factor=lf)
@@ -93,11 +93,11 @@ def test_autotune_then_tuned_calls_match(engine, tmp_path):
         engine.set_tuning(None)
         store.free()
     d = engine.tuning()
-    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, 8, -1, -1, 64)
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, -1, -1, -1, 64)
 
 
 def test_tuning_out_of_range_is_rejected(engine):
-    for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0),
+    for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0), dict(filter_blocks=-2),
                 dict(align_blocks=65), dict(align_blocks=-2), dict(fork_shapes=2), dict(fork_shapes=-2), dict(list_factor=0),
                 dict(win_period=17), dict(win_period=-1), dict(win_period=3), dict(shape_mask=4096), dict(shape_bias=4),
                 dict(split_rows=63), dict(split_rows=32769), dict(split_rows=-1)]:
@@ -110,7 +110,7 @@ def test_tuning_out_of_range_is_rejected(engine):
         with pytest.raises(gsdr.GsdrError):
             engine.set_tuning(tu)
     d = engine.tuning()
-    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, 8, -1, -1, 64)
+    assert (d.swap_ratio, d.spec_max, d.filter_blocks, d.align_blocks, d.fork_shapes, d.list_factor) == (4, 64, -1, -1, -1, 64)
 
 
 @pytest.mark.parametrize("law,n,t", [(4, 5000, 0.95), (4, 4000, 0.80), (2, 3000, 0.99), (1, 2000, 0.90)])
