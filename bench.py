@@ -294,7 +294,7 @@ def main():
     ap.add_argument("--pigeon", action="store_true", help="enable the exact pigeonhole prefilter")
     ap.add_argument("--no-screen", action="store_true", help="turn the word-signature screen off")
     ap.add_argument("--split-rows", type=int, default=-1,
-                    help="phase-A pairs with this many rows or more on two waves (0 = never; default: built in)")
+                    help="pairs with this many rows or more on two waves (0 = never; default: built in)")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="set one scalar launch knob of gsdr_tuning (repeatable), e.g. align_blocks=8")
     ap.add_argument("--serial-shapes", action="store_true",
@@ -369,7 +369,7 @@ def main():
         eng.set_tuning(tu)
         tuning = dict(tuning or {}, screen=0)
     if args.split_rows >= 0:
-        # the two-wave (forward and backward halves) phase-A pairs
+        # pairs with this many rows on two waves (forward and backward halves)
         tu = eng.tuning()
         tu.split_rows = args.split_rows
         eng.set_tuning(tu)

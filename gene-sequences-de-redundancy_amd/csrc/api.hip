@@ -359,7 +359,7 @@ int gsdr_set_tuning(gsdr_ctx* c, const gsdr_tuning* t) {
       t->filter_blocks > 64 || t->align_blocks < -1 || t->align_blocks > 64 ||
       t->fork_shapes < -1 || t->fork_shapes > 1 || t->list_factor < 1 || t->list_factor > 256 ||
       (t->pigeon != 0 && t->pigeon != 1) || (t->screen != 0 && t->screen != 1) || t->win_period < 0 ||
-      t->win_period > 16 || (t->win_period & (t->win_period - 1)) != 0 || (t->split_rows != 0 && (t->split_rows < 64 || t->split_rows > 32768)))
+      t->win_period > 16 || (t->win_period & (t->win_period - 1)) != 0 || (t->split_rows != 0 && t->split_rows != -1 && (t->split_rows < 64 || t->split_rows > 32768)))
     return fail(GSDR_E_INVALID, "tuning out of range");
   for (int b = 0; b < kLenBins; ++b)
     if (t->shape_mask[b] < 0 || (uint32_t)t->shape_mask[b] > kAllShapes || t->shape_bias[b] < 0 ||

@@ -22,7 +22,8 @@ namespace gsdr {
 // One step for every job. Phase A: the speculative representatives against
 // each other, then resolve_reps fixes the true ones. Phase B: filter,
 // alignment (skipping candidates an earlier true representative already
-// took), resolve. The tile-shape kernels of a phase run concurrently.
+// took), resolve. The tile-shape kernels of a phase run concurrently or one
+// after another, with the split kernel first when long pairs are split.
 // ev (optional) brackets filter and alignment time of both phases.
 hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent_t* ev) {
   hipError_t e;
@@ -40,7 +41,9 @@ hipError_t launch_step(gsdr_ctx* c, const StepLaunch& L, hipStream_t q, hipEvent
   if ((e = launch_filter(L.fa_main, L.kmask, L.filter_grid, q)) != hipSuccess) return e;
   if (L.pigeon && (e = launch_pigeon(L.pa, L.pigeon_grid, q)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[4], q)) != hipSuccess) return e;
-  if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork)) != hipSuccess) return e;
+  if ((e = launch_align_shapes(c, L.aa, L.smax, L.align_grid, q, L.fork, L.split ? &L.aa_split : nullptr,
+                               L.split_grid)) != hipSuccess)
+    return e;
   if (ev && (e = hipEventRecord(ev[5], q)) != hipSuccess) return e;
   if ((e = launch_resolve(L.ra, q)) != hipSuccess) return e;
   // with several jobs, bound the next step's widths by the lists' capacity
@@ -80,16 +83,18 @@ StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax
   L.fa_main.screen = L.screen;
   L.screen_grid = c->cu_count * 8;
   L.sa = ScreenArgs{w.jobs, njobs, w.stats};
+  // the split kernel's resident blocks: 4 per CU (two waves each, at most
+  // two waves per SIMD by its registers); the split list takes one pair per
+  // block, so every split pair starts at once. Both phases use the list:
+  // the resolve kernels clear its counters between them.
+  const int split_rows = split_rows_for(T, mode);
+  L.split = split_rows > 0;
+  L.split_grid = c->cu_count * 4;
+  L.fa_main.split_rows = split_rows;
+  L.fa_main.split_cap = std::min<int64_t>(w.split_cap, L.split_grid);
   L.fa_reps = L.fa_main;
   L.fa_reps.phase = PHASE_REPS;
   L.fa_reps.screen = 0;
-  L.split = T.split_rows > 0;
-  L.fa_reps.split_rows = T.split_rows;
-  // the split kernel's resident blocks: 4 per CU (two waves each, at most
-  // two waves per SIMD by its registers); the split list takes one pair per
-  // block, so every split pair starts at once
-  L.split_grid = c->cu_count * 4;
-  L.fa_reps.split_cap = std::min<int64_t>(w.split_cap, L.split_grid);
   L.aa_split.jobs = w.jobs;
   L.aa_split.items = w.items + (int64_t)kSplitList * w.item_cap;
   L.aa_split.count = w.counters + kSplitList;

@@ -374,10 +374,10 @@ int align_pairs(gsdr_ctx* c, const char* data, const int64_t* data_off, int nseq
     if (cells_exec_out) *cells_exec_out = 0;
     return GSDR_OK;
   }
-  // shape 12 + s: split pairs starting from 64-lane shape s (4..11)
+  // shape 12 + s: split pairs starting from the window of shape s (0..11;
+  // at least two words per lane)
   const bool split = shape >= kNumShapes;
-  const int min_shape = shape < 0 ? 0 : split ? std::min(std::max(shape - kNumShapes, 4), kNumShapes - 1)
-                                              : std::min(shape, kNumShapes - 1);
+  const int min_shape = shape < 0 ? 0 : std::min(split ? shape - kNumShapes : shape, kNumShapes - 1);
   // a private store: sequences, plus one query slot per pair (nseq + pair)
   // that points at the query's packed words and carries the pair's cutoff
   const int64_t nslots = (int64_t)nseq + npairs;

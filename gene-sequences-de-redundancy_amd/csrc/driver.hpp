@@ -43,7 +43,7 @@ struct Tuning {
   int screen = 1;        // word-signature screen before the phase-B filter (k >= 6)
   gsdr::ShapeTable shapes = {};  // tile shapes per text-length bin (zero: the narrowest that holds the band)
   int win_period = 0;    // 64-lane tiles: blocks between DP-edge window rebases (0 = by width)
-  int split_rows = 0;    // phase-A pairs with this many rows run on two waves (0 = never)
+  int split_rows = -1;   // pairs with this many rows run on two waves (0 = never, -1 = by the call)
 };
 
 struct gsdr_ctx {
@@ -182,8 +182,8 @@ struct StepLaunch {
   ResolveArgs ra;
   AlignArgs aa_reps[kNumShapes];
   AlignArgs aa[kNumShapes];
-  AlignArgs aa_split;  // phase A's split list
-  int split = 0;       // launch the split kernel in phase A
+  AlignArgs aa_split;  // the split list (both phases)
+  int split = 0;       // launch the split kernel in both phases
   int split_grid = 1;
   int smax = 0;
   unsigned kmask = 0;  // word lengths among the jobs (bit k)
@@ -252,6 +252,17 @@ inline int filter_blocks_for(const Tuning& T, GridMode m) {
 // Tile-shape kernels concurrent on side streams (1) or one after another (0).
 inline int fork_for(const Tuning& T, GridMode m) {
   return T.fork_shapes >= 0 ? T.fork_shapes : (m == GridMode::kLatency ? 0 : 1);
+}
+// Rows from which a pair is split over two waves: a knob of 0 or 64..32768
+// is the value; -1 is 128 in the latency form, where a step ends with its
+// longest pair's rows (the 177,000-record t=0.99 job 580 -> 550 ms per
+// clustering, 10,000 records t=0.95 26.1 -> 24.4 ms), and never otherwise
+// (2,000,000 records t=0.95 367 -> 377 ms with 2,048, 200,000 records
+// t=0.90 493 -> 501 ms with 4,096: pairs past the split list's room
+// contend for its counter in the filter before they go to their shapes).
+constexpr int kLatencySplitRows = 128;
+inline int split_rows_for(const Tuning& T, GridMode m) {
+  return T.split_rows >= 0 ? T.split_rows : (m == GridMode::kLatency ? kLatencySplitRows : 0);
 }
 StepLaunch make_step_launch(gsdr_ctx* c, const Workspace& w, int njobs, int smax, unsigned kmask,
                             int32_t* jobs_left, GridMode mode, const Tuning& T = Tuning());

@@ -19,8 +19,8 @@ struct FilterArgs {
   unsigned long long* stats;
   int32_t screen;           // phase B: skip what the signature screen rejected
   ShapeTable shapes = {};   // tile-shape choice per text-length bin (zero: narrowest of all)
-  int32_t split_rows = 0;   // phase A: pairs of 64-lane tiles with this many rows or more go to the
-                            // split list (0 = none)
+  int32_t split_rows = 0;   // pairs with this many rows or more go to the split list (0 = none;
+                            // at least 64, so that each half holds 32 rows)
   int64_t split_cap = 0;    // pairs the split list takes (it follows the kNumShapes shape lists);
                             // later ones go to their shape lists
 };
@@ -461,9 +461,9 @@ __global__ __launch_bounds__(256, filter_waves_per_eu(G)) void filter_kernel(Fil
         it.query = sw ? r : q;
         it.job = job;
         it.slot = slot | (sw ? kSwapFlag : 0);
-        // long phase-A pairs of the 64-lane tiles: two waves each (the
-        // split list), starting from the window of their shape
-        if (a.phase == PHASE_REPS && a.split_rows > 0 && sh >= 4 && (sw ? n : mq) >= a.split_rows) {
+        // long pairs: two waves each (the split list), starting from the
+        // window of their shape (at least two words per lane)
+        if (a.split_rows > 0 && (sw ? n : mq) >= a.split_rows) {
           it.slot |= sh << kSplitShapeShift;
           sh = kSplitList;
         }

@@ -95,7 +95,7 @@ int autotune(gsdr_ctx* c, gsdr_store& s, const gsdr_params& p0, int max_trials, 
       {&Tuning::align_blocks, {2, 4, 8, 0}},
       {&Tuning::filter_blocks, {4, 8, 16, 32}},
       {&Tuning::fork_shapes, {0, 1}},
-      {&Tuning::split_rows, {0, 1024}},
+      {&Tuning::split_rows, {0, 128, 1024}},
   };
   for (const Knob& k : knobs) {
     for (int v : k.values) {

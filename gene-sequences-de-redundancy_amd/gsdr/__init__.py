@@ -94,6 +94,7 @@ class Tuning(C.Structure):
         kw.setdefault("align_blocks", -1)
         kw.setdefault("filter_blocks", -1)
         kw.setdefault("fork_shapes", -1)
+        kw.setdefault("split_rows", -1)
         super().__init__(**kw)
 
     def as_dict(self):

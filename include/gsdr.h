@@ -119,10 +119,13 @@ typedef struct gsdr_tuning {
   int32_t shape_bias[4];
   int32_t win_period;     /* 64-lane tiles: 32-row blocks between window rebases at the DP edge
                              (0 = 2 with 6 or more words per lane, 4 with fewer; else 1, 2, 4, 8 or 16) */
-  int32_t split_rows;     /* batched: a pair of the speculative representatives against each other
-                             on a 64-lane tile with at least this many rows is aligned by two waves,
-                             half of the rows forward and half backward, joined at the middle row
-                             (default 0 = never; else 64..32768) */
+  int32_t split_rows;     /* batched: a pair with at least this many rows (query bases) is aligned by
+                             two waves, half of the rows forward and half backward, joined at the
+                             middle row, while the split list has room (one pair per block of the
+                             split kernel; the rest run on their tile shapes)
+                             (0 = never; -1 = by the call: 128 where every threshold of the call
+                             is at least 0.99 or no pair needs more than 4 words per lane, never
+                             otherwise; else 64..32768) */
 } gsdr_tuning;
 
 /* ---- options (host side): replaces checkOption (func.cpp:6-57) and the
@@ -254,9 +257,10 @@ int gsdr_get_tuning(const gsdr_ctx *ctx, gsdr_tuning *out);
  *      (func.cpp:572-676). Sequences are given as raw bytes like FASTA data;
  *      pairs holds npairs (text index, query index) pairs. shape < 0 picks the
  *      tile shape per pair; 0..11 forces tile shape index `shape` (or the
- *      narrowest wider one that holds the pair); 12 + s (s = 4..11) runs every
+ *      narrowest wider one that holds the pair); 12 + s (s = 0..11) runs every
  *      pair with at least 64 query bases as a split pair (two waves, joined at
- *      the middle row) starting from shape s or wider. ---- */
+ *      the middle row) starting from the window of shape s or wider, with at
+ *      least two words per lane. ---- */
 int gsdr_align_pairs(gsdr_ctx *ctx, const char *data, const int64_t *data_off, int32_t nseq,
                      const int32_t *pairs, const int32_t *bc, int32_t npairs, int32_t shape,
                      int32_t *accept_out, double *cells_exec_out);

@@ -187,8 +187,10 @@ def test_empty_and_degenerate(engine):
 # at the middle row; pairs with fewer than 64 query bases run unsplit
 
 
-@pytest.mark.parametrize("shape", [4, 5, 6, 7, 9, 11])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7, 9, 11])
 def test_split_boundary_decisions(engine, shape):
+    """Starting shapes 0..3 (one word per lane) start the halves from two
+    words per lane: the pairs whose bands a one-word tile holds."""
     seqs, pairs = make_pairs(200 + shape, 80, 5000)
     ep, bcs, expect = boundary_cases(seqs, pairs)
     got, _ = engine.align_pairs(seqs, ep, bcs, shape=12 + shape)
@@ -196,7 +198,7 @@ def test_split_boundary_decisions(engine, shape):
     assert len(bad) == 0, f"{len(bad)} mismatches, first: pair {ep[bad[0]]} bc {bcs[bad[0]]} expect {expect[bad[0]]}"
 
 
-@pytest.mark.parametrize("shape", [4, 8])
+@pytest.mark.parametrize("shape", [0, 4, 8])
 def test_split_single_base_runs(engine, shape):
     """Single-base-run texts through both halves (carries ripple through every
     word on the reversed pair too), queries of every length class."""

@@ -100,7 +100,7 @@ def test_tuning_out_of_range_is_rejected(engine):
     for bad in [dict(swap_ratio=-1), dict(spec_max=0), dict(spec_max=65), dict(filter_blocks=0), dict(filter_blocks=-2),
                 dict(align_blocks=65), dict(align_blocks=-2), dict(fork_shapes=2), dict(fork_shapes=-2), dict(list_factor=0),
                 dict(win_period=17), dict(win_period=-1), dict(win_period=3), dict(shape_mask=4096), dict(shape_bias=4),
-                dict(split_rows=63), dict(split_rows=32769), dict(split_rows=-1)]:
+                dict(split_rows=63), dict(split_rows=32769), dict(split_rows=-2)]:
         tu = gsdr.Tuning(swap_ratio=4, spec_max=64, filter_blocks=8, align_blocks=4, fork_shapes=1, list_factor=64)
         for k, v in bad.items():
             if k in ("shape_mask", "shape_bias"):
@@ -115,8 +115,9 @@ def test_tuning_out_of_range_is_rejected(engine):
 
 @pytest.mark.parametrize("law,n,t", [(4, 5000, 0.95), (4, 4000, 0.80), (2, 3000, 0.99), (1, 2000, 0.90)])
 def test_split_pairs_give_the_oracle_clusters(engine, tmp_path, law, n, t):
-    """Phase-A pairs aligned by two waves (forward and backward halves joined
-    at the middle row) from 64 rows up, and never: the oracle's clusters."""
+    """Pairs of both phases aligned by two waves (forward and backward halves
+    joined at the middle row) from 64 rows up, from 300 rows up, and never:
+    the oracle's clusters."""
     r, ocl = _oracle(tmp_path, gsdr.Records.generate(law, 1200 + law, n), t)
     store = engine.load(r, gsdr.auto_word_length(t))
     try:
