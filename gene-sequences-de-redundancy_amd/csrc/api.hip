@@ -185,7 +185,7 @@ int gsdr_records_sort(gsdr_records* r, int64_t* perm_out) {
   if (!r) return fail(GSDR_E_INVALID, "null records");
   Records& rec = r->r;
   if (!rec.length_ordered) {
-    std::vector<int64_t> perm = length_order(rec);
+    std::vector<int64_t> perm = length_order(rec, host_threads());
     std::vector<int64_t> prior = std::move(rec.order);
     rec = permute(rec, perm, host_threads());
     if (!prior.empty())

@@ -63,33 +63,7 @@ struct CodeTable {
 static const CodeTable kTable;
 static inline uint8_t base_code(unsigned char c) { return kTable.v[c]; }
 
-// ------------------------------------------------------------------- order
-// func.cpp:105-107: std::sort(reads, [](a, b) { return a.data.size() >
-// b.data.size(); }). libstdc++'s introsort makes the same moves for any
-// element type given the same comparison results, so sorting (length, index)
-// keys with the same comparator yields the reference's permutation,
-// including its (unstable) order of equal lengths.
 namespace {
-template <class I>
-std::vector<int64_t> sort_keys(const std::vector<int64_t>& lens) {
-  struct Key { I len; I idx; };
-  const int64_t n = (int64_t)lens.size();
-  std::vector<Key> keys((size_t)n);
-  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{(I)lens[(size_t)i], (I)i};
-  std::sort(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; });
-  std::vector<int64_t> perm((size_t)n);
-  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int64_t)keys[(size_t)i].idx;
-  return perm;
-}
-
-// The moves depend only on the comparison results, so 32-bit keys (when
-// lengths and indices fit) give the same permutation with half the traffic.
-std::vector<int64_t> order_by_length(const std::vector<int64_t>& lens) {
-  const int64_t n = (int64_t)lens.size();
-  bool narrow = n <= INT32_MAX;
-  for (int64_t i = 0; narrow && i < n; ++i) narrow = lens[(size_t)i] <= INT32_MAX;
-  return narrow ? sort_keys<int32_t>(lens) : sort_keys<int64_t>(lens);
-}
 
 // Runs body(lo, hi) over [0, n) in chunks handed out dynamically.
 template <class F>
@@ -108,6 +82,89 @@ void parallel_chunks(int64_t n, int64_t chunk, int threads, F body) {
   for (auto& x : th) x.join();
 }
 
+// ------------------------------------------------------------------- order
+// func.cpp:105-107: std::sort(reads, [](a, b) { return a.data.size() >
+// b.data.size(); }). libstdc++'s introsort makes the same moves for any
+// element type given the same comparison results, so sorting (length, index)
+// keys with the same comparator yields the reference's permutation,
+// including its (unstable) order of equal lengths.
+//
+// std::sort is __introsort_loop (median-of-three partitions until a range
+// holds 16 elements or the depth limit 2*lg(n) is spent, then heapsort)
+// followed by one insertion pass over the whole array. This runs the same
+// steps on threads: the partitions of the top levels (each range's
+// partition touches only that range, so the order in which disjoint ranges
+// are partitioned changes no move), then __introsort_loop on each resulting
+// range with its remaining depth, then the insertion pass per range. The
+// insertion pass never moves an element across a partition point (every
+// element left of one is ordered before or equal to every element right of
+// it), and insertion sort is stable, so per range it gives the arrangement
+// of the whole pass.
+template <class It, class Cmp>
+void sort_like_std(It first, It last, Cmp comp, int threads) {
+  const ptrdiff_t n = last - first;
+  if (n < 2) return;
+  const auto c = __gnu_cxx::__ops::__iter_comp_iter(comp);
+  struct Range {
+    It f, l;
+    ptrdiff_t depth;
+  };
+  const ptrdiff_t grain = std::max<ptrdiff_t>(ptrdiff_t(1) << 15, n / (8 * std::max(threads, 1)));
+  std::vector<Range> level{Range{first, last, (ptrdiff_t)std::__lg(n) * 2}}, leaves;
+  while (!level.empty()) {
+    std::vector<Range> kids(2 * level.size());
+    std::vector<char> split(level.size(), 0);
+    parallel_chunks((int64_t)level.size(), 1, threads, [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) {
+        const Range r = level[(size_t)i];
+        if (r.l - r.f <= grain || r.depth == 0) continue;
+        const It cut = std::__unguarded_partition_pivot(r.f, r.l, c);
+        kids[2 * (size_t)i] = Range{r.f, cut, r.depth - 1};
+        kids[2 * (size_t)i + 1] = Range{cut, r.l, r.depth - 1};
+        split[(size_t)i] = 1;
+      }
+    });
+    std::vector<Range> next;
+    for (size_t i = 0; i < level.size(); ++i) {
+      if (split[i]) {
+        next.push_back(kids[2 * i]);
+        next.push_back(kids[2 * i + 1]);
+      } else {
+        leaves.push_back(level[i]);
+      }
+    }
+    level.swap(next);
+  }
+  parallel_chunks((int64_t)leaves.size(), 1, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+      const Range r = leaves[(size_t)i];
+      std::__introsort_loop(r.f, r.l, r.depth, c);
+      std::__insertion_sort(r.f, r.l, c);
+    }
+  });
+}
+
+template <class I>
+std::vector<int64_t> sort_keys(const std::vector<int64_t>& lens, int threads) {
+  struct Key { I len; I idx; };
+  const int64_t n = (int64_t)lens.size();
+  std::vector<Key> keys((size_t)n);
+  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{(I)lens[(size_t)i], (I)i};
+  sort_like_std(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; }, threads);
+  std::vector<int64_t> perm((size_t)n);
+  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int64_t)keys[(size_t)i].idx;
+  return perm;
+}
+
+// The moves depend only on the comparison results, so 32-bit keys (when
+// lengths and indices fit) give the same permutation with half the traffic.
+std::vector<int64_t> order_by_length(const std::vector<int64_t>& lens, int threads) {
+  const int64_t n = (int64_t)lens.size();
+  bool narrow = n <= INT32_MAX;
+  for (int64_t i = 0; narrow && i < n; ++i) narrow = lens[(size_t)i] <= INT32_MAX;
+  return narrow ? sort_keys<int32_t>(lens, threads) : sort_keys<int64_t>(lens, threads);
+}
+
 // Copies the bytes of [p, e) other than '\n' to dst; returns the bytes written.
 int64_t copy_lines(const char* p, const char* e, char* dst) {
   char* d = dst;
@@ -123,11 +180,11 @@ int64_t copy_lines(const char* p, const char* e, char* dst) {
 
 }  // namespace
 
-std::vector<int64_t> length_order(const Records& r) {
+std::vector<int64_t> length_order(const Records& r, int threads) {
   const int64_t n = r.count();
   std::vector<int64_t> lens((size_t)n);
   for (int64_t i = 0; i < n; ++i) lens[(size_t)i] = r.len(i);
-  return order_by_length(lens);
+  return order_by_length(lens, threads);
 }
 
 // ------------------------------------------------------------------ reader
@@ -230,7 +287,7 @@ int read_fasta(const char* path, Records& out, std::string& err, int threads, bo
   });
 
   std::vector<int64_t> perm;
-  if (length_order) perm = order_by_length(dlen);
+  if (length_order) perm = order_by_length(dlen, threads);
   // records are copied in file order (sequential reads of the file) to
   // their place in the output order
   std::vector<int64_t> slot;

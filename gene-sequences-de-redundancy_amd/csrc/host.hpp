@@ -66,8 +66,9 @@ struct Records {
 int read_fasta(const char* path, Records& out, std::string& err, int threads = 1, bool length_order = false);
 
 // Stable-free length order of the reference (func.cpp:105-107): libstdc++
-// std::sort with "longer first". Returns perm with perm[sorted] = original.
-std::vector<int64_t> length_order(const Records& r);
+// std::sort with "longer first", on `threads` threads with the same result.
+// Returns perm with perm[sorted] = original.
+std::vector<int64_t> length_order(const Records& r, int threads = 1);
 Records permute(const Records& r, const std::vector<int64_t>& perm, int threads = 1);
 
 // Writes representatives (cluster[s] == s) in sorted order, unwrapped

@@ -86,6 +86,30 @@ def test_fasta_roundtrip_and_sort(tmp_path):
     assert np.array_equal(perm, O.sort_order(lens))
 
 
+@pytest.mark.parametrize("pattern", ["ties", "pareto", "equal", "saw", "down", "pipe"])
+def test_threaded_length_order_matches_std_sort(pattern):
+    """The length order is std::sort's, unstable tie order included, with
+    the top partitions and the per-range sorts on threads: 300,000 records
+    (several levels of partitions) against the oracle's std::sort over
+    std::string records."""
+    n = 300_000
+    rng = np.random.default_rng(11)
+    i = np.arange(n)
+    lens = {
+        "ties": rng.integers(0, 41, n),
+        "pareto": np.minimum((10 * rng.random(n) ** (-1 / 1.5)).astype(np.int64), 3000),
+        "equal": np.full(n, 5),
+        "saw": i % 97,
+        "down": (n - i) % 50,
+        "pipe": np.minimum(i, n - i) % 200,
+    }[pattern].astype(np.int64)
+    r = gsdr.Records.from_lists([b"A" * int(x) for x in lens])
+    try:
+        assert np.array_equal(r.sort(), O.sort_order(lens))
+    finally:
+        r.free()
+
+
 def test_parallel_reader_awkward_file(tmp_path):
     """A file of several MiB read in parallel slices: ragged line widths,
     blank lines, CRLF, '>' inside data lines, a header right after a header,
