@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -92,9 +94,9 @@ void parallel_chunks(int64_t n, int64_t chunk, int threads, F body) {
 // std::sort is __introsort_loop (median-of-three partitions until a range
 // holds 16 elements or the depth limit 2*lg(n) is spent, then heapsort)
 // followed by one insertion pass over the whole array. This runs the same
-// steps on threads: the partitions of the top levels (each range's
-// partition touches only that range, so the order in which disjoint ranges
-// are partitioned changes no move), then __introsort_loop on each resulting
+// steps on threads: the partitions of large ranges (each range's partition
+// touches only that range, so the order in which disjoint ranges are
+// partitioned changes no move), then __introsort_loop on each resulting
 // range with its remaining depth, then the insertion pass per range. The
 // insertion pass never moves an element across a partition point (every
 // element left of one is ordered before or equal to every element right of
@@ -110,59 +112,70 @@ void sort_like_std(It first, It last, Cmp comp, int threads) {
     ptrdiff_t depth;
   };
   const ptrdiff_t grain = std::max<ptrdiff_t>(ptrdiff_t(1) << 15, n / (8 * std::max(threads, 1)));
-  std::vector<Range> level{Range{first, last, (ptrdiff_t)std::__lg(n) * 2}}, leaves;
-  while (!level.empty()) {
-    std::vector<Range> kids(2 * level.size());
-    std::vector<char> split(level.size(), 0);
-    parallel_chunks((int64_t)level.size(), 1, threads, [&](int64_t lo, int64_t hi) {
-      for (int64_t i = lo; i < hi; ++i) {
-        const Range r = level[(size_t)i];
-        if (r.l - r.f <= grain || r.depth == 0) continue;
-        const It cut = std::__unguarded_partition_pivot(r.f, r.l, c);
-        kids[2 * (size_t)i] = Range{r.f, cut, r.depth - 1};
-        kids[2 * (size_t)i + 1] = Range{cut, r.l, r.depth - 1};
-        split[(size_t)i] = 1;
-      }
-    });
-    std::vector<Range> next;
-    for (size_t i = 0; i < level.size(); ++i) {
-      if (split[i]) {
-        next.push_back(kids[2 * i]);
-        next.push_back(kids[2 * i + 1]);
-      } else {
-        leaves.push_back(level[i]);
-      }
-    }
-    level.swap(next);
+  if (threads <= 1 || n <= grain) {
+    std::sort(first, last, comp);
+    return;
   }
-  parallel_chunks((int64_t)leaves.size(), 1, threads, [&](int64_t lo, int64_t hi) {
-    for (int64_t i = lo; i < hi; ++i) {
-      const Range r = leaves[(size_t)i];
-      std::__introsort_loop(r.f, r.l, r.depth, c);
-      std::__insertion_sort(r.f, r.l, c);
+  // A shared stack of ranges: a thread takes one, partitions it and pushes
+  // both parts while it is large and has depth left, else finishes it.
+  // Ranges never wait for one another, so lopsided partitions of heavily
+  // tied lengths do not hold up the rest.
+  std::vector<Range> stack{Range{first, last, (ptrdiff_t)std::__lg(n) * 2}};
+  std::mutex mu;
+  std::condition_variable cv;
+  int busy = 0;
+  auto work = [&]() {
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+      cv.wait(lk, [&] { return !stack.empty() || busy == 0; });
+      if (stack.empty()) break;  // and no thread can push more
+      const Range r = stack.back();
+      stack.pop_back();
+      ++busy;
+      lk.unlock();
+      if (r.l - r.f > grain && r.depth > 0) {
+        const It cut = std::__unguarded_partition_pivot(r.f, r.l, c);
+        lk.lock();
+        stack.push_back(Range{r.f, cut, r.depth - 1});
+        stack.push_back(Range{cut, r.l, r.depth - 1});
+        --busy;
+        cv.notify_all();
+      } else {
+        std::__introsort_loop(r.f, r.l, r.depth, c);
+        std::__insertion_sort(r.f, r.l, c);
+        lk.lock();
+        --busy;
+        if (stack.empty() && busy == 0) cv.notify_all();
+      }
     }
-  });
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < threads; ++t) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
 }
 
 template <class I>
-std::vector<int64_t> sort_keys(const std::vector<int64_t>& lens, int threads) {
+std::vector<int64_t> sort_keys(const int64_t* lens, int64_t n, int threads) {
   struct Key { I len; I idx; };
-  const int64_t n = (int64_t)lens.size();
-  std::vector<Key> keys((size_t)n);
-  for (int64_t i = 0; i < n; ++i) keys[(size_t)i] = Key{(I)lens[(size_t)i], (I)i};
+  std::vector<Key, NoInitAlloc<Key>> keys((size_t)n);
+  parallel_chunks(n, 1 << 16, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) keys[(size_t)i] = Key{(I)lens[i], (I)i};
+  });
   sort_like_std(keys.begin(), keys.end(), [](Key& a, Key& b) { return a.len > b.len; }, threads);
   std::vector<int64_t> perm((size_t)n);
-  for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int64_t)keys[(size_t)i].idx;
+  parallel_chunks(n, 1 << 16, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) perm[(size_t)i] = (int64_t)keys[(size_t)i].idx;
+  });
   return perm;
 }
 
 // The moves depend only on the comparison results, so 32-bit keys (when
 // lengths and indices fit) give the same permutation with half the traffic.
-std::vector<int64_t> order_by_length(const std::vector<int64_t>& lens, int threads) {
-  const int64_t n = (int64_t)lens.size();
+std::vector<int64_t> order_by_length(const int64_t* lens, int64_t n, int threads) {
   bool narrow = n <= INT32_MAX;
-  for (int64_t i = 0; narrow && i < n; ++i) narrow = lens[(size_t)i] <= INT32_MAX;
-  return narrow ? sort_keys<int32_t>(lens, threads) : sort_keys<int64_t>(lens, threads);
+  for (int64_t i = 0; narrow && i < n; ++i) narrow = lens[i] <= INT32_MAX;
+  return narrow ? sort_keys<int32_t>(lens, n, threads) : sort_keys<int64_t>(lens, n, threads);
 }
 
 // Copies the bytes of [p, e) other than '\n' to dst; returns the bytes written.
@@ -184,7 +197,7 @@ std::vector<int64_t> length_order(const Records& r, int threads) {
   const int64_t n = r.count();
   std::vector<int64_t> lens((size_t)n);
   for (int64_t i = 0; i < n; ++i) lens[(size_t)i] = r.len(i);
-  return order_by_length(lens, threads);
+  return order_by_length(lens.data(), n, threads);
 }
 
 // ------------------------------------------------------------------ reader
@@ -256,7 +269,7 @@ int read_fasta(const char* path, Records& out, std::string& err, int threads, bo
   std::vector<int64_t> first((size_t)nslice + 1, 0);
   for (int64_t sl = 0; sl < nslice; ++sl) first[(size_t)sl + 1] = first[(size_t)sl] + (int64_t)part[(size_t)sl].start.size();
   const int64_t n = first[(size_t)nslice];
-  std::vector<int64_t> start((size_t)n + 1), hdr((size_t)n), dlen((size_t)n);
+  Index start((size_t)n + 1), hdr((size_t)n), dlen((size_t)n);
   parallel_chunks(nslice, 1, threads, [&](int64_t lo, int64_t hi) {
     for (int64_t sl = lo; sl < hi; ++sl) {
       Slice& v = part[(size_t)sl];
@@ -287,18 +300,21 @@ int read_fasta(const char* path, Records& out, std::string& err, int threads, bo
   });
 
   std::vector<int64_t> perm;
-  if (length_order) perm = order_by_length(dlen, threads);
+  if (length_order) perm = order_by_length(dlen.data(), n, threads);
   // records are copied in file order (sequential reads of the file) to
   // their place in the output order
-  std::vector<int64_t> slot;
+  Index slot;
   if (length_order) {
     slot.resize((size_t)n);
     parallel_chunks(n, 65536, threads, [&](int64_t lo, int64_t hi) {
       for (int64_t i = lo; i < hi; ++i) slot[(size_t)perm[(size_t)i]] = i;
     });
   }
-  out.name_off.assign((size_t)n + 1, 0);
-  out.data_off.assign((size_t)n + 1, 0);
+  // every entry past the first is written below
+  out.name_off.resize((size_t)n + 1);
+  out.data_off.resize((size_t)n + 1);
+  out.name_off[0] = 0;
+  out.data_off[0] = 0;
   parallel_chunks(n, 8192, threads, [&](int64_t lo, int64_t hi) {
     for (int64_t r = lo; r < hi; ++r) {
       const int64_t i = length_order ? slot[(size_t)r] : r;

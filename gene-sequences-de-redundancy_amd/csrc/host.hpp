@@ -35,12 +35,15 @@ struct NoInitAlloc {
   template <class U> bool operator!=(const NoInitAlloc<U>&) const noexcept { return false; }
 };
 using Bytes = std::vector<char, NoInitAlloc<char>>;
+// Per-record offsets: sized without zeroing, first touched by the threads
+// that fill them.
+using Index = std::vector<int64_t, NoInitAlloc<int64_t>>;
 
 struct Records {
   Bytes names;                      // concatenated header bytes
-  std::vector<int64_t> name_off;    // size n+1
+  Index name_off;                   // size n+1
   Bytes data;                       // concatenated raw sequence bytes
-  std::vector<int64_t> data_off;    // size n+1
+  Index data_off;                   // size n+1
   // Set once the records are in the reference's length order; `order` then
   // holds the input position of each record.
   bool length_ordered = false;
