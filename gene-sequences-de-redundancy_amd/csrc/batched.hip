@@ -210,8 +210,8 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
   const int64_t step_limit = max_n + 2;
   // kernels per step: two filters, resolve_reps and the three resolve
   // kernels, the shape kernels of both phases, the clamp with several jobs,
-  // the screen and the pigeonhole tests when on, the split kernel of phase A
-  const int64_t per_step = 6 + 2 * (L.smax + 1) + (njobs > 1 ? 1 : 0) + L.screen + 2 * L.pigeon + L.split;
+  // the screen and the pigeonhole tests when on, the split kernel of each phase
+  const int64_t per_step = 6 + 2 * (L.smax + 1) + (njobs > 1 ? 1 : 0) + L.screen + 2 * L.pigeon + 2 * L.split;
   if (timing) {
     // timed form: the same step launched directly with events around the
     // filter and the alignment kernels
@@ -312,7 +312,7 @@ int cluster_batched(gsdr_ctx* c, int njobs, gsdr_store* const* stores, const gsd
     stats->ms_filter = ms_filter;
     stats->ms_align = ms_align;
     stats->filter_launches = (2 + L.screen) * steps_launched;
-    stats->align_launches = steps_launched * 2 * (L.smax + 1);
+    stats->align_launches = steps_launched * 2 * (L.smax + 1 + L.split);
   }
   return GSDR_OK;
 }
