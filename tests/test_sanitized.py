@@ -134,6 +134,47 @@ def test_reader_and_writers_sanitized(tmp_path, host_driver, name, threads):
     assert [i for i, _ in got] == O.sort_order([len(d) for _, d in recs]).tolist()
 
 
+def _big_fasta(path):
+    """200,000 records of 1..53 bases: enough to split the length order over
+    threads, with long runs of equal lengths."""
+    path.write_bytes(b"".join(b">b%d\n" % i + b"ACGT"[i % 4:i % 4 + 1] * (i * 7919 % 53 + 1) + b"\n"
+                              for i in range(200_000)))
+
+
+@pytest.mark.parametrize("threads", [3, 16])
+def test_threaded_length_order_sanitized(tmp_path, host_driver, threads):
+    """The threaded length order (partitions from a shared task stack) under
+    ASan/UBSan: the driver checks it against the one-thread std::sort order."""
+    import oracle_ffi as O
+
+    fa = tmp_path / "big.fa"
+    _big_fasta(fa)
+    r = subprocess.run([str(host_driver), str(fa), str(threads)], capture_output=True, timeout=600, env=ENV)
+    assert r.returncode == 0, (r.returncode, r.stderr.decode(errors="replace")[-3000:])
+    assert b"runtime error" not in r.stderr and b"AddressSanitizer" not in r.stderr
+    got = [int(x.split()[0]) for x in r.stdout.decode().splitlines()[:-1]]
+    lens = [i * 7919 % 53 + 1 for i in range(200_000)]
+    assert got == O.sort_order(lens).tolist()
+
+
+def test_threaded_length_order_thread_sanitizer(tmp_path):
+    """The same reader and length order built with ThreadSanitizer: no data
+    race between the threads that partition and finish the ranges."""
+    exe = tmp_path / "tsan_driver"
+    cpp = tmp_path / "tsan_driver.cpp"
+    cpp.write_text(DRIVER)
+    r = subprocess.run(["g++", "-std=c++17", "-fsanitize=thread", "-g", "-O1", "-I", CSRC, str(cpp),
+                        os.path.join(CSRC, "host.cpp"), "-o", str(exe), "-lpthread"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    fa = tmp_path / "big.fa"
+    _big_fasta(fa)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([str(exe), str(fa), "8"], capture_output=True, timeout=900, env=env)
+    assert r.returncode == 0, (r.returncode, r.stderr.decode(errors="replace")[-3000:])
+    assert b"ThreadSanitizer" not in r.stderr
+
+
 def test_generator_and_preprocessing_sanitized(tmp_path):
     exe = build(tmp_path, "gen", GEN, [os.path.join(CSRC, "host.cpp")])
     r = subprocess.run([str(exe)], capture_output=True, timeout=600, env=ENV)
