@@ -258,6 +258,27 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
              (double)prop.totalGlobalMem / (double)(1ll << 30));
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return fail(GSDR_E_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+  // The first symbol lookup loads the library's code object (tens of ms);
+  // it runs on a helper thread, with the device current there, beside the
+  // stream and event creation below, and is joined before the context is
+  // handed out (also on every early return).
+  struct Loader {
+    int st = GSDR_OK;
+    std::string err;
+    void* ptr = nullptr;
+    std::thread t;
+    ~Loader() { if (t.joinable()) t.join(); }
+  } ld;
+  ld.t = std::thread([&ld, device] {
+    if (hipSetDevice(device) != hipSuccess) {
+      ld.st = GSDR_E_DEVICE;
+      ld.err = "hipSetDevice (code object load)";
+      return;
+    }
+    // this device's copy of the no-allocation scratch array
+    ld.st = scratch_setup(device, &ld.ptr);
+    if (ld.st != GSDR_OK) ld.err = gsdr_last_error();
+  });
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipHostMalloc((void**)&c->h_state, sizeof(JobState) * 4, hipHostMallocDefault), "hipHostMalloc");
@@ -267,9 +288,9 @@ int gsdr_open(int32_t device, gsdr_ctx** out) {
     HIP_TRY(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming), "hipEventCreate");
   }
   HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate");
-  // this device's copy of the no-allocation scratch array (device current)
-  const int sst = scratch_setup(device, &c->scratch);
-  if (sst != GSDR_OK) return sst;
+  ld.t.join();
+  if (ld.st != GSDR_OK) return fail(ld.st, ld.err);
+  c->scratch = ld.ptr;
   *out = c.release();
   return GSDR_OK;
 }

@@ -57,9 +57,11 @@ int main(int argc, char** argv) {
   gsdr_ctx* ctx = nullptr;
   int open_st = GSDR_OK;
   double open_s = 0.0;
+  std::string open_err;  // the library's error text is per thread
   std::thread opener([&] {
     const auto a = std::chrono::steady_clock::now();
     open_st = gsdr_open(o.device, &ctx);
+    if (open_st) open_err = gsdr_last_error();
     open_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
   });
 
@@ -73,7 +75,10 @@ int main(int argc, char** argv) {
   if (!st) st = gsdr_records_sort(recs, perm.data());
   const auto t_sort = std::chrono::steady_clock::now();
   opener.join();
-  if (open_st) return die("open device", open_st);
+  if (open_st) {
+    std::cerr << "cluster: open device: " << gsdr_strerror(open_st) << ": " << open_err << std::endl;
+    return 1;
+  }
   if (st) return die(recs ? "sort" : "read", st);
   char name[256];
   gsdr_device_info(ctx, name, sizeof(name), nullptr, nullptr);
